@@ -1,0 +1,195 @@
+// capi.hip -- C ABI plumbing: errors, memory, set-up and target geometry.
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include <string>
+
+#include "common.hpp"
+#include "healpix.hpp"
+
+namespace hcg {
+namespace {
+thread_local std::string g_last_error;
+}
+
+int set_error(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+}  // namespace hcg
+
+using namespace hcg;
+
+extern "C" const char* hcg_strerror(int code) {
+  switch (code) {
+    case HCG_OK: return "ok";
+    case HCG_ERR_INVALID: return "invalid argument";
+    case HCG_ERR_HIP: return "HIP runtime error";
+    case HCG_ERR_NOMEM: return "out of memory";
+    case HCG_ERR_UNSUPPORTED: return "unsupported size or parameter";
+    case HCG_ERR_WORKSPACE: return "workspace too small";
+    case HCG_ERR_IO: return "I/O or FITS format error";
+    default: return "unknown error";
+  }
+}
+
+extern "C" const char* hcg_last_error_message(void) { return g_last_error.c_str(); }
+
+extern "C" int hcg_device_count(int* count) {
+  HCG_REQUIRE(count != nullptr, "hcg_device_count: NULL");
+  HCG_CHECK_HIP(hipGetDeviceCount(count));
+  return HCG_OK;
+}
+
+extern "C" int hcg_set_device(int device) {
+  HCG_CHECK_HIP(hipSetDevice(device));
+  return HCG_OK;
+}
+
+extern "C" int hcg_malloc(void** dptr, size_t bytes) {
+  HCG_REQUIRE(dptr != nullptr, "hcg_malloc: NULL");
+  *dptr = nullptr;
+  if (bytes == 0) return HCG_OK;
+  hipError_t e = hipMalloc(dptr, bytes);
+  if (e != hipSuccess) {
+    *dptr = nullptr;
+    return set_error(HCG_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  }
+  return HCG_OK;
+}
+
+extern "C" int hcg_free(void* dptr) {
+  if (dptr) HCG_CHECK_HIP(hipFree(dptr));
+  return HCG_OK;
+}
+
+extern "C" int hcg_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream) {
+  if (bytes == 0) return HCG_OK;
+  HCG_REQUIRE(dst && src, "hcg_memcpy_h2d: NULL");
+  HCG_CHECK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, as_stream(stream)));
+  return HCG_OK;
+}
+
+extern "C" int hcg_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream) {
+  if (bytes == 0) return HCG_OK;
+  HCG_REQUIRE(dst && src, "hcg_memcpy_d2h: NULL");
+  HCG_CHECK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, as_stream(stream)));
+  return HCG_OK;
+}
+
+extern "C" int hcg_memset(void* dst, int value, size_t bytes, void* stream) {
+  if (bytes == 0) return HCG_OK;
+  HCG_REQUIRE(dst != nullptr, "hcg_memset: NULL");
+  HCG_CHECK_HIP(hipMemsetAsync(dst, value, bytes, as_stream(stream)));
+  return HCG_OK;
+}
+
+extern "C" int hcg_stream_create(void** stream) {
+  HCG_REQUIRE(stream != nullptr, "hcg_stream_create: NULL");
+  hipStream_t s;
+  HCG_CHECK_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  *stream = s;
+  return HCG_OK;
+}
+
+extern "C" int hcg_stream_destroy(void* stream) {
+  if (stream) HCG_CHECK_HIP(hipStreamDestroy(as_stream(stream)));
+  return HCG_OK;
+}
+
+extern "C" int hcg_stream_synchronize(void* stream) {
+  HCG_CHECK_HIP(hipStreamSynchronize(as_stream(stream)));
+  return HCG_OK;
+}
+
+// _Healpix_init + _update_params (HCGrid/healpix.cu:42-65).
+extern "C" int hcg_healpix_init(uint64_t nside, hcg_healpix* hp) {
+  HCG_REQUIRE(hp != nullptr, "hcg_healpix_init: NULL");
+  HCG_REQUIRE(nside >= 1 && (nside & (nside - 1)) == 0 && nside <= (1ull << 29),
+              "hcg_healpix_init: nside must be a power of two");
+  hp->nside = nside;
+  hp->order = ilog2_u64(nside);
+  hp->nrings = 4 * nside - 1;
+  hp->npface = nside * nside;
+  hp->ncap = (hp->npface - nside) << 1;
+  hp->npix = 12 * hp->npface;
+  hp->fact2 = 4. / (double)hp->npix;
+  hp->fact1 = (double)(nside << 1) * hp->fact2;
+  hp->omega = 4. * kPi / (double)hp->npix;
+  hp->resolution = sqrt(hp->omega);
+  return HCG_OK;
+}
+
+// HCGrid/helpers.cu:98-102
+extern "C" uint64_t hcg_set_optimal_nside(double target_res) {
+  uint64_t nside = (uint64_t)(sqrt(kPi / 3.) / target_res + 0.5);
+  return (uint64_t)1 << (ilog2_u64(nside) + 1);
+}
+
+// HCGrid/gmap.cpp:64-99
+extern "C" int hcg_prepare_grid_kernel(uint32_t kernel_type, const double* kernel_params, double sphere_radius,
+                                       double hpx_max_res, hcg_kernel* kern, hcg_healpix* hp) {
+  HCG_REQUIRE(kernel_params && kern && hp, "hcg_prepare_grid_kernel: NULL argument");
+  HCG_REQUIRE(sphere_radius > 0 && hpx_max_res > 0, "hcg_prepare_grid_kernel: radius and resolution must be > 0");
+  hcg_kernel k{};
+  k.kernel_type = kernel_type;
+  k.factor = 1;  // the caller sets the coarsening factor afterwards (HCGrid/HCGrid.cpp:158-161)
+  if (kernel_type == HCG_GAUSS1D) {
+    HCG_REQUIRE(kernel_params[0] > 0, "hcg_prepare_grid_kernel: sigma must be > 0");
+    k.params[0] = 0.5 / pow(kernel_params[0], 2.);
+    k.bearing_needed = 0;
+  } else if (kernel_type == HCG_GAUSS2D) {
+    for (int i = 0; i < 3; ++i) k.params[i] = kernel_params[i];
+    k.bearing_needed = 1;
+  } else if (kernel_type == HCG_TAPERED_SINC) {
+    for (int i = 0; i < 3; ++i) k.params[i] = kernel_params[i];
+    k.bearing_needed = 0;
+  } else {
+    return set_error(HCG_ERR_INVALID, "hcg_prepare_grid_kernel: unknown kernel type");
+  }
+  const uint64_t nside = hcg_set_optimal_nside(kDeg2Rad * hpx_max_res);
+  int rc = hcg_healpix_init(nside, hp);
+  if (rc != HCG_OK) return rc;
+  k.sphere_radius = sphere_radius;
+  k.disc_size = (kDeg2Rad * sphere_radius + hp->resolution);
+  *kern = k;
+  return HCG_OK;
+}
+
+// SIN deprojection (Calabretta & Greisen 2002, Paper II sec. 5.1.5, PV2_1 =
+// PV2_2 = 0) and the native -> celestial rotation (Paper II eq. 2) with
+// (alpha_p, delta_p) = CRVAL and phi_p = LONPOLE.  Cell (x, y) <- pixel
+// (x+1, y+1) as set_WCS builds pixcrd (HCGrid/gmap.cpp:209-238).
+extern "C" int hcg_wcs_sin_p2s(uint32_t naxis1, uint32_t naxis2, double crpix1, double crpix2, double cdelt1,
+                               double cdelt2, double crval1, double crval2, double lonpole, double* xwcs,
+                               double* ywcs) {
+  HCG_REQUIRE(xwcs && ywcs, "hcg_wcs_sin_p2s: NULL output");
+  const double a0 = crval1 * kDeg2Rad, d0 = crval2 * kDeg2Rad, php = lonpole * kDeg2Rad;
+  const double sd0 = sin(d0), cd0 = cos(d0);
+  for (uint32_t yy = 0; yy < naxis2; ++yy) {
+    const double y = cdelt2 * ((double)(yy + 1) - crpix2) * kDeg2Rad;
+    for (uint32_t xx = 0; xx < naxis1; ++xx) {
+      const double x = cdelt1 * ((double)(xx + 1) - crpix1) * kDeg2Rad;
+      const double r2 = x * x + y * y;
+      const uint64_t c = (uint64_t)yy * naxis1 + xx;
+      if (r2 > 1.0) {  // outside the projection's boundary
+        xwcs[c] = NAN;
+        ywcs[c] = NAN;
+        continue;
+      }
+      // native latitude: cos(theta) = r, sin(theta) = sqrt(1 - r^2)
+      const double cth = sqrt(r2), sth = sqrt(1.0 - r2);
+      const double phi = atan2(x, -y);
+      const double dphi = phi - php;
+      const double sdp = sin(dphi), cdp = cos(dphi);
+      const double sdec = sth * sd0 + cth * cd0 * cdp;
+      const double dec = asin(sdec > 1.0 ? 1.0 : (sdec < -1.0 ? -1.0 : sdec));
+      double ra = (a0 + atan2(-cth * sdp, sth * cd0 - cth * sd0 * cdp)) * kRad2Deg;
+      ra = fmod(ra, 360.0);
+      if (ra < 0) ra += 360.0;
+      xwcs[c] = ra;
+      ywcs[c] = dec * kRad2Deg;
+    }
+  }
+  return HCG_OK;
+}

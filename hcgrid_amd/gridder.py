@@ -1,0 +1,201 @@
+"""Host-side driver of the device gridding path (Python over the C ABI).
+
+Mirrors the reference's host sequence of solve_gridding
+(HCGrid/gridding.cu:484-556): kernel set-up (_prepare_grid_kernel), the
+pre-sort and ring LUT (init_input_with_cpu), zeroed output cubes
+(init_output), the gridding kernel and the output normalisation
+(write_output_map) -- with every array resident in HBM.  torch is only the
+device allocator and stream provider; all compute is in libhcgrid_amd.so.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import capi
+from .capi import GAUSS1D, Healpix, Kernel, RingMeta, call
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def prepare_grid_kernel(kernel_type: int, kernel_params, sphere_radius: float, hpx_max_res: float,
+                        factor: int = 1):
+    """_prepare_grid_kernel (HCGrid/gmap.cpp:64-99) -> (Kernel, Healpix)."""
+    capi.load()
+    params = (C.c_double * 3)(*([float(x) for x in kernel_params] + [0.0] * (3 - len(kernel_params))))
+    kern = Kernel()
+    hp = Healpix()
+    call("hcg_prepare_grid_kernel", kernel_type, params, float(sphere_radius), float(hpx_max_res),
+         C.byref(kern), C.byref(hp))
+    kern.factor = int(factor)
+    return kern, hp
+
+
+def wcs_sin(side_x: int, side_y: int, crpix1: float, crpix2: float, cdelt1: float, cdelt2: float,
+            crval1: float, crval2: float, lonpole: float = 180.0):
+    """Target cell coordinates (set_WCS, HCGrid/gmap.cpp:183-250), host arrays."""
+    capi.load()
+    xw = np.empty(side_x * side_y)
+    yw = np.empty(side_x * side_y)
+    call("hcg_wcs_sin_p2s", side_x, side_y, crpix1, crpix2, cdelt1, cdelt2, crval1, crval2, lonpole,
+         xw.ctypes.data_as(C.c_void_p), yw.ctypes.data_as(C.c_void_p))
+    return xw, yw
+
+
+def _ptr(t) -> C.c_void_p | None:
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+def _stream_handle(stream) -> C.c_void_p:
+    torch = _torch()
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return C.c_void_p(s.cuda_stream)
+
+
+@dataclass
+class PlainBuffers:
+    keys: object        # int32 view of uint32 [n+1]
+    perm: object        # int32 view of uint32 [n]
+    workspace: object   # uint8
+    records: object     # float64 [n*4]
+    lon_s: object
+    lat_s: object
+    w_s: object
+    lut: object         # int32 view of uint32 [nrings+2]
+    ring_z: object
+    ring_s: object
+    meta: object        # int32 [4]
+    datacube: object    # float64 [side*side]
+    weightscube: object
+    map: object
+
+
+class PlainGridder:
+    """Single-channel drop-in path with resident buffers for (n, side, nside)."""
+
+    def __init__(self, n: int, side: int, kern: Kernel, hp: Healpix, device: int = 0,
+                 with_weights: bool = False):
+        torch = _torch()
+        capi.load()
+        if not torch.cuda.is_available():
+            raise capi.HcgError(capi.HCG_ERR_HIP, "PlainGridder", "no GPU visible to torch")
+        self.n, self.side, self.kern, self.hp = int(n), int(side), kern, hp
+        dev = torch.device("cuda", device)
+        self.device = dev
+        i32, f64 = torch.int32, torch.float64
+        ws_bytes = capi.load().hcg_presort_workspace_size(self.n)
+        nr = int(hp.nrings) + 2
+        m = self.side * self.side
+        self.buf = PlainBuffers(
+            keys=torch.empty(self.n + 1, dtype=i32, device=dev),
+            perm=torch.empty(max(self.n, 1), dtype=i32, device=dev),
+            workspace=torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev),
+            records=torch.empty(max(self.n, 1) * 4, dtype=f64, device=dev),
+            lon_s=torch.empty(max(self.n, 1), dtype=f64, device=dev),
+            lat_s=torch.empty(max(self.n, 1), dtype=f64, device=dev),
+            w_s=torch.empty(max(self.n, 1), dtype=f64, device=dev) if with_weights else None,
+            lut=torch.empty(nr, dtype=i32, device=dev),
+            ring_z=torch.empty(nr, dtype=f64, device=dev),
+            ring_s=torch.empty(nr, dtype=f64, device=dev),
+            meta=torch.zeros(4, dtype=i32, device=dev),
+            datacube=torch.empty(max(m, 1), dtype=f64, device=dev),
+            weightscube=torch.empty(max(m, 1), dtype=f64, device=dev),
+            map=torch.empty(max(m, 1), dtype=f64, device=dev),
+        )
+        self.ws_bytes = ws_bytes
+
+    # -- stages ------------------------------------------------------------
+    def presort(self, lon, lat, stream=None):
+        b = self.buf
+        call("hcg_presort", C.byref(self.hp), self.n, _ptr(lon), _ptr(lat), _ptr(b.keys), _ptr(b.perm),
+             _ptr(b.workspace), self.ws_bytes, _stream_handle(stream))
+
+    def gather(self, lon, lat, data, weights=None, stream=None):
+        torch = _torch()
+        b = self.buf
+        dtype = capi.DTYPE_F64 if data.dtype == torch.float64 else capi.DTYPE_F32
+        if data.dtype not in (torch.float64, torch.float32):
+            raise capi.HcgError(capi.HCG_ERR_INVALID, "gather", "data must be float32 or float64")
+        if weights is not None and b.w_s is None:
+            raise capi.HcgError(capi.HCG_ERR_INVALID, "gather", "gridder built without weights")
+        call("hcg_gather_samples", self.n, _ptr(b.perm), _ptr(lon), _ptr(lat), _ptr(data), dtype,
+             _ptr(weights), _ptr(b.records), _ptr(b.lon_s), _ptr(b.lat_s), None,
+             _ptr(b.w_s) if weights is not None else None, _stream_handle(stream))
+
+    def ring_lut(self, stream=None):
+        b = self.buf
+        call("hcg_ring_lut", C.byref(self.hp), self.n, _ptr(b.keys), _ptr(b.lut), _ptr(b.ring_z),
+             _ptr(b.ring_s), _ptr(b.meta), _stream_handle(stream))
+
+    def init_output(self, stream=None):
+        b = self.buf
+        m = self.side * self.side * 8
+        sh = _stream_handle(stream)
+        call("hcg_memset", _ptr(b.datacube), 0, m, sh)
+        call("hcg_memset", _ptr(b.weightscube), 0, m, sh)
+
+    def grid(self, xwcs, ywcs, weighted=False, stream=None):
+        b = self.buf
+        call("hcg_grid_plain", C.byref(self.hp), C.byref(self.kern), self.side, self.n, _ptr(b.records),
+             _ptr(b.keys), _ptr(b.lon_s), _ptr(b.lat_s), _ptr(b.w_s) if weighted else None, _ptr(b.lut),
+             _ptr(b.ring_z), _ptr(b.ring_s), _ptr(b.meta), _ptr(xwcs), _ptr(ywcs), _ptr(b.datacube),
+             _ptr(b.weightscube), _stream_handle(stream))
+
+    def normalize(self, stream=None):
+        b = self.buf
+        call("hcg_normalize", self.side * self.side, _ptr(b.datacube), _ptr(b.weightscube), _ptr(b.map),
+             _stream_handle(stream))
+
+    def run(self, lon, lat, data, xwcs, ywcs, weights=None, stream=None, normalize=True):
+        """One pass of the hot path; everything stays on the device."""
+        self.presort(lon, lat, stream)
+        self.gather(lon, lat, data, weights, stream)
+        self.ring_lut(stream)
+        self.init_output(stream)
+        self.grid(xwcs, ywcs, weighted=weights is not None, stream=stream)
+        if normalize:
+            self.normalize(stream)
+
+    # -- results -------------------------------------------------------------
+    def meta(self) -> dict:
+        m = self.buf.meta.cpu().numpy().view(np.uint32)
+        return dict(firstring=int(m[0]), usedrings=int(m[1]), n_valid=int(m[2]), n=int(m[3]))
+
+    def keys_sorted(self) -> np.ndarray:
+        return self.buf.keys.cpu().numpy().view(np.uint32).copy()
+
+    def perm(self) -> np.ndarray:
+        return self.buf.perm[: self.n].cpu().numpy().view(np.uint32).copy()
+
+    def lut(self) -> np.ndarray:
+        used = self.meta()["usedrings"]
+        return self.buf.lut[: used + 1].cpu().numpy().view(np.uint32).copy()
+
+
+def grid_plain(lon, lat, data, xwcs, ywcs, side: int, sigma: float, radius: float, hpx_res: float,
+               weights=None, factor: int = 1, kernel_type: int = GAUSS1D, kernel_params=None,
+               device: int = 0) -> dict:
+    """Host arrays in, host arrays out: one full device pass (for tests and tools)."""
+    torch = _torch()
+    if kernel_params is None:
+        kernel_params = [sigma]
+    kern, hp = prepare_grid_kernel(kernel_type, kernel_params, radius, hpx_res, factor)
+    g = PlainGridder(len(lon), side, kern, hp, device=device, with_weights=weights is not None)
+    dev = g.device
+    t = lambda a, dt: torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device=dev)  # noqa: E731
+    lon_d, lat_d = t(lon, torch.float64), t(lat, torch.float64)
+    data_np = np.asarray(data)
+    data_d = t(data_np, torch.float64 if data_np.dtype == np.float64 else torch.float32)
+    xw_d, yw_d = t(xwcs, torch.float64), t(ywcs, torch.float64)
+    w_d = None if weights is None else t(weights, torch.float64)
+    g.run(lon_d, lat_d, data_d, xw_d, yw_d, weights=w_d)
+    torch.cuda.synchronize()
+    out = dict(meta=g.meta(), keys_sorted=g.keys_sorted(), perm=g.perm(), lut=g.lut(),
+               datacube=g.buf.datacube.cpu().numpy().copy(), weightscube=g.buf.weightscube.cpu().numpy().copy(),
+               map=g.buf.map.cpu().numpy().copy(), kern=kern, hp=hp)
+    return out

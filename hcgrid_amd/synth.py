@@ -1,0 +1,111 @@
+"""Seeded synthetic raster-scan workloads (numpy).
+
+Raster scans: constant-Dec scan lines, samples evenly spaced in RA plus a
+uniform jitter; values are Gaussian point sources (a 0.025 deg FWHM beam)
+plus white noise.  Sky coordinates are FP64 degrees.  The target is a square
+SIN-projection map.  Generated with numpy's PCG64 so that every run with a
+given seed produces the same bytes.
+
+Named workloads:
+  raster_250k  : 200 lines x 1250 samples over 4 x 4 deg at (180, 30);
+                 400 x 400 map, 0.01 deg pixels; sigma 0.02 deg, R = 3 sigma.
+  raster_20m   : 1000 lines x 20000 samples over 20 x 20 deg at (180, 30);
+                 2048 x 2048 map, 0.01 deg pixels; same kernel.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+
+BEAM_FWHM_DEG = 0.025
+BEAM_SIGMA_DEG = BEAM_FWHM_DEG / math.sqrt(8.0 * math.log(2.0))
+
+
+@dataclasses.dataclass
+class Target:
+    side: int            # NAXIS1 = NAXIS2
+    cdelt: float         # deg, |CDELT1| = CDELT2
+    crval1: float
+    crval2: float
+    crpix: float         # CRPIX1 = CRPIX2 (the reference generator's N/2)
+
+    def header_cards(self) -> dict:
+        return {
+            "NAXIS": 3, "NAXIS1": self.side, "NAXIS2": self.side, "NAXIS3": 1,
+            "CTYPE1": "RA---SIN", "CTYPE2": "DEC--SIN", "CUNIT1": "deg", "CUNIT2": "deg",
+            "CDELT1": -self.cdelt, "CDELT2": self.cdelt, "CRPIX1": self.crpix,
+            "CRPIX2": self.crpix, "CRVAL1": self.crval1, "CRVAL2": self.crval2,
+        }
+
+
+@dataclasses.dataclass
+class Workload:
+    name: str
+    lon: np.ndarray      # f64 deg
+    lat: np.ndarray      # f64 deg
+    values: np.ndarray   # f32
+    target: Target
+    sigma: float = 0.02
+    radius: float = 0.06
+    hpx_res: float = 0.01
+
+    @property
+    def n(self) -> int:
+        return int(self.lon.size)
+
+
+def _haversine_deg(l1, b1, l2, b2):
+    d2r = math.pi / 180.0
+    l1, b1, l2, b2 = (np.asarray(v, dtype=np.float64) * d2r for v in (l1, b1, l2, b2))
+    a = np.sin((b1 - b2) / 2.0) ** 2 + np.cos(b1) * np.cos(b2) * np.sin((l1 - l2) / 2.0) ** 2
+    return 2.0 * np.arcsin(np.sqrt(a)) / d2r
+
+
+def raster(n_lines: int, n_per_line: int, dec0: float, ddec: float, ra0: float, dra: float,
+           jitter: float, n_src: int, seed: int, noise: float = 1.0):
+    """Raster-scan coordinates and single-channel values."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    k = np.arange(n_lines, dtype=np.float64)
+    j = np.arange(n_per_line, dtype=np.float64)
+    lat = np.repeat(dec0 + ddec * k, n_per_line)
+    lon = np.tile(ra0 + dra * j, n_lines) + rng.uniform(-jitter, jitter, size=n_lines * n_per_line)
+    lo_ra, hi_ra = ra0, ra0 + dra * (n_per_line - 1)
+    lo_de, hi_de = dec0, dec0 + ddec * (n_lines - 1)
+    amp = rng.uniform(1.0, 10.0, size=n_src)
+    sra = rng.uniform(lo_ra, hi_ra, size=n_src)
+    sde = rng.uniform(lo_de, hi_de, size=n_src)
+    values = (noise * rng.standard_normal(lon.size)).astype(np.float64)
+    reach = 6.0 * BEAM_SIGMA_DEG
+    for a, r, d in zip(amp, sra, sde):
+        # only lines within reach of the source can be touched
+        rows = np.nonzero(np.abs(dec0 + ddec * k - d) < reach)[0]
+        if rows.size == 0:
+            continue
+        sl = slice(int(rows[0]) * n_per_line, (int(rows[-1]) + 1) * n_per_line)
+        dist = _haversine_deg(lon[sl], lat[sl], r, d)
+        values[sl] += a * np.exp(-dist * dist / (2.0 * BEAM_SIGMA_DEG ** 2))
+    return lon, lat, values.astype(np.float32), (amp, sra, sde)
+
+
+def raster_250k(seed: int = 0) -> Workload:
+    lon, lat, val, _ = raster(200, 1250, 28.01, 0.02, 178.0016, 0.0032, 0.0005, 20, seed)
+    return Workload("raster_250k", lon, lat, val, Target(400, 0.01, 180.0, 30.0, 200.0))
+
+
+def raster_20m(seed: int = 1) -> Workload:
+    lon, lat, val, _ = raster(1000, 20000, 20.01, 0.02, 170.0005, 0.001, 0.0005, 20, seed)
+    return Workload("raster_20m", lon, lat, val, Target(2048, 0.01, 180.0, 30.0, 1024.0))
+
+
+def raster_small(n_lines: int, n_per_line: int, side: int, dec_c: float = 30.0, ra_c: float = 180.0,
+                 spacing_dec: float = 0.02, spacing_ra: float = 0.0032, cdelt: float = 0.01,
+                 seed: int = 7, n_src: int = 5) -> Workload:
+    """A small raster centred on (ra_c, dec_c), sized for CPU-speed tests."""
+    dec0 = dec_c - spacing_dec * (n_lines - 1) / 2.0
+    ra0 = ra_c - spacing_ra * (n_per_line - 1) / 2.0
+    lon, lat, val, _ = raster(n_lines, n_per_line, dec0, spacing_dec, ra0, spacing_ra,
+                              spacing_ra / 6.4, n_src, seed)
+    return Workload(f"raster_{n_lines}x{n_per_line}", lon, lat, val,
+                    Target(side, cdelt, ra_c, dec_c, side / 2.0))

@@ -1,0 +1,185 @@
+/*
+ * hcgrid_amd.h -- C ABI of the MI355X (gfx950) gridding library.
+ *
+ * This is the only layer that talks to HIP.  Everything is plain C: integer
+ * sizes, plain pointers, opaque stream handles (a hipStream_t passed as void*,
+ * NULL = the default stream).  Functions return HCG_OK (0) or a negative
+ * error code; none of them calls exit().  hcg_strerror() names a code and
+ * hcg_last_error_message() carries the detail of the calling thread's last
+ * failure.
+ *
+ * Unless a parameter says "host", pointers are device pointers and the call
+ * only enqueues work on `stream` (no synchronisation).  The caller owns every
+ * buffer; functions that need scratch take a workspace whose size a *_size()
+ * query gives.
+ *
+ * Each entry point names the reference code it replaces (paths relative to
+ * the HCGrid repository).
+ */
+#ifndef HCGRID_AMD_H
+#define HCGRID_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes ---- */
+#define HCG_OK                0
+#define HCG_ERR_INVALID      -1   /* bad argument (NULL, size, unsupported kernel type) */
+#define HCG_ERR_HIP          -2   /* a HIP runtime call failed */
+#define HCG_ERR_NOMEM        -3   /* allocation failed */
+#define HCG_ERR_UNSUPPORTED  -4   /* nside or size beyond what the device path supports */
+#define HCG_ERR_WORKSPACE    -5   /* workspace smaller than the *_size() query */
+#define HCG_ERR_IO           -6   /* file I/O or FITS format error */
+
+/* ---- constants (HCGrid/constants.h:57-73) ---- */
+#define HCG_MAX_Y          1073741824u  /* key of a sample whose theta is outside [0, pi] */
+#define HCG_GAUSS1D        1u
+#define HCG_GAUSS2D        2u
+#define HCG_TAPERED_SINC   3u
+#define HCG_NESTED         1u
+#define HCG_RING           2u
+#define HCG_DTYPE_F64      0
+#define HCG_DTYPE_F32      1
+#define HCG_DTYPE_BF16     2
+#define HCG_DTYPE_F16      3
+#define HCG_MAX_NSIDE      8192u  /* keys are uint32; npix(8192) < MAX_Y, as the reference needs */
+
+/* HEALPix geometry: the fields of `struct Healpix` (HCGrid/healpix.h:14-21)
+ * that the path reads.  Filled by hcg_healpix_init(). */
+typedef struct hcg_healpix {
+    uint64_t nside, order, nrings, npface, ncap, npix;
+    double fact1, fact2, omega, resolution;
+} hcg_healpix;
+
+/* Gridding-kernel state left by _prepare_grid_kernel (HCGrid/gmap.h:47-61,
+ * HCGrid/gmap.cpp:64-99): kernel type, transformed parameters, radius and the
+ * disc size that bounds the ring and longitude windows. */
+typedef struct hcg_kernel {
+    uint32_t kernel_type;     /* HCG_GAUSS1D / HCG_GAUSS2D / HCG_TAPERED_SINC */
+    uint32_t bearing_needed;  /* 1 for GAUSS2D */
+    uint32_t factor;          /* coarsening factor gamma: cells per window anchor (>=1) */
+    uint32_t reserved;
+    double params[3];         /* GAUSS1D: params[0] = 0.5 / sigma^2 (sigma in degrees) */
+    double sphere_radius;     /* degrees */
+    double disc_size;         /* radians: DEG2RAD * sphere_radius + resolution */
+} hcg_kernel;
+
+/* Sorted-sample record the gridding kernels read: unit vector of the sample
+ * direction and its value, 32 bytes, one per sorted sample. */
+typedef struct hcg_sample {
+    double x, y, z, v;
+} hcg_sample;
+
+/* Device-side summary written by hcg_ring_lut(). */
+typedef struct hcg_ring_meta {
+    uint32_t firstring;  /* ring of the first sorted sample */
+    uint32_t usedrings;  /* rings spanned by the valid samples (0 if none) */
+    uint32_t n_valid;    /* samples with a key < npix (invalid ones sort last) */
+    uint32_t n;          /* all samples */
+} hcg_ring_meta;
+
+/* ---- errors and device ---- */
+const char *hcg_strerror(int code);
+const char *hcg_last_error_message(void);
+int hcg_device_count(int *count);                        /* host out */
+int hcg_set_device(int device);
+
+/* Memory plumbing for callers without their own allocator (the C++ host layer). */
+int hcg_malloc(void **dptr, size_t bytes);
+int hcg_free(void *dptr);
+int hcg_memcpy_h2d(void *dst, const void *host_src, size_t bytes, void *stream);
+int hcg_memcpy_d2h(void *host_dst, const void *src, size_t bytes, void *stream);
+int hcg_memset(void *dst, int value, size_t bytes, void *stream);
+int hcg_stream_create(void **stream);                    /* default priority */
+int hcg_stream_destroy(void *stream);
+int hcg_stream_synchronize(void *stream);
+
+/* ---- set-up, host only ---- */
+/* _Healpix_init + _update_params (HCGrid/healpix.cu:42-65).  RING only. */
+int hcg_healpix_init(uint64_t nside, hcg_healpix *hp);
+/* set_optimal_nside (HCGrid/helpers.cu:98-102); target resolution in radians. */
+uint64_t hcg_set_optimal_nside(double target_res_rad);
+/* _prepare_grid_kernel (HCGrid/gmap.cpp:64-99).  kernel_params (host, 3
+ * doubles) is read, not modified; the transformed values land in kern->params.
+ * hpx_max_res_deg fixes nside.  Returns HCG_ERR_INVALID for unknown types. */
+int hcg_prepare_grid_kernel(uint32_t kernel_type, const double *kernel_params,
+                            double sphere_radius_deg, double hpx_max_res_deg,
+                            hcg_kernel *kern, hcg_healpix *hp);
+
+/* ---- the device path ---- */
+
+/* Workspace bytes hcg_presort() needs for n samples. */
+size_t hcg_presort_workspace_size(uint32_t n);
+
+/* HEALPix keys + stable sort: the ang2pix loop and sort of
+ * init_input_with_cpu (HCGrid/gridding.cu:70-100; init_input_with_thrust,
+ * :149-168).  Keys are RING pixels of (theta, phi) = (HALFPI - DEG2RAD*lat,
+ * DEG2RAD*lon), HCG_MAX_Y for theta outside [0, pi].  Output is stable by
+ * (key, original index), the order of sort_param 2 (parallel_stable_sort),
+ * for every caller.  keys_sorted holds n+1 entries; entry n is the npix
+ * sentinel (HCGrid/gridding.cu:100).  perm[i] = original index of slot i. */
+int hcg_presort(const hcg_healpix *hp, uint32_t n, const double *lon_deg, const double *lat_deg,
+                uint32_t *keys_sorted, uint32_t *perm, void *workspace, size_t workspace_bytes,
+                void *stream);
+
+/* The three permutation gathers of init_input_with_cpu (HCGrid/gridding.cu:101-114),
+ * fused, plus the sample records the gridding kernel reads.  data has
+ * data_dtype (HCG_DTYPE_F64 or HCG_DTYPE_F32).  weights may be NULL (all 1,
+ * as read_input_map sets them, HCGrid/gmap.cpp:140-142).  Any of lon_sorted,
+ * lat_sorted, data_sorted, weights_sorted may be NULL to skip that output;
+ * records must not be NULL. */
+int hcg_gather_samples(uint32_t n, const uint32_t *perm, const double *lon_deg,
+                       const double *lat_deg, const void *data, int data_dtype,
+                       const double *weights, hcg_sample *records, double *lon_sorted,
+                       double *lat_sorted, double *data_sorted, double *weights_sorted,
+                       void *stream);
+
+/* Two-level ring lookup table, level 1 (HCGrid/gridding.cu:116-136), built on
+ * the device.  lut[k] = first sorted index whose key >= startpix(firstring+k)
+ * for 0 < k < usedrings, lut[0] = 0, lut[usedrings] = n.  ring_z/ring_s hold
+ * cos/sin of the centre colatitude of each used ring (what the kernel's
+ * pix2ang(startpix) + ang2pix recompute per cell, HCGrid/gridding.cu:332-351).
+ * Capacity: lut and ring_z/ring_s need hp->nrings + 2 entries.  meta (device)
+ * receives firstring/usedrings/n_valid. */
+int hcg_ring_lut(const hcg_healpix *hp, uint32_t n, const uint32_t *keys_sorted, uint32_t *lut,
+                 double *ring_z, double *ring_s, hcg_ring_meta *meta, void *stream);
+
+/* The `hcgrid` kernel (HCGrid/gridding.cu:277-403): for each of side*side
+ * cells (row length `side` = NAXIS2, HCGrid/gmap.cpp:174; cell c =
+ * row*side + col), accumulate w*v and w into datacube/weightscube in place
+ * over the reference's candidate set: rings [uring, dring] of the window
+ * anchor (the first cell of each `factor` group), skipping the first used
+ * ring, sorted indices with key in [lpix_r, rpix_r] (whole ring when the
+ * anchor's colatitude is <= 20 or >= 160 degrees), and great-circle distance
+ * < sphere_radius.  Weight w = weights_sorted[i] * kernel(d) (weights_sorted
+ * may be NULL = 1).  FP64 geometry and accumulation.  lon/lat_sorted are
+ * needed only for GAUSS2D bearings and distances at the radius boundary. */
+int hcg_grid_plain(const hcg_healpix *hp, const hcg_kernel *kern, uint32_t side,
+                   uint32_t n, const hcg_sample *records, const uint32_t *keys_sorted,
+                   const double *lon_sorted, const double *lat_sorted,
+                   const double *weights_sorted, const uint32_t *lut, const double *ring_z,
+                   const double *ring_s, const hcg_ring_meta *meta, const double *xwcs,
+                   const double *ywcs, double *datacube, double *weightscube, void *stream);
+
+/* Output normalisation of write_output_map (HCGrid/gmap.cpp:305-312):
+ * map[c] = datacube[c] / weightscube[c] in FP64, NaN where the weight is 0.
+ * map may alias datacube. */
+int hcg_normalize(uint64_t ncells, const double *datacube, const double *weightscube,
+                  double *map, void *stream);
+
+/* ---- host-side target geometry ---- */
+/* set_WCS for a SIN (orthographic) target (HCGrid/gmap.cpp:183-250): cell
+ * (x, y) at pixel (x+1, y+1) -> world (lon, lat) degrees, cell index
+ * y*naxis1 + x.  Host pointers.  Closed form of Calabretta & Greisen 2002. */
+int hcg_wcs_sin_p2s(uint32_t naxis1, uint32_t naxis2, double crpix1, double crpix2,
+                    double cdelt1, double cdelt2, double crval1, double crval2,
+                    double lonpole, double *xwcs, double *ywcs);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HCGRID_AMD_H */

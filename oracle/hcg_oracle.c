@@ -1,0 +1,620 @@
+/*
+ * hcg_oracle.c -- CPU restatement of HCGrid's gridding path (the oracle).
+ *
+ * TEST INFRASTRUCTURE ONLY: loaded by tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg as the checker.  Never part of the product.
+ *
+ * Build: gcc -O2 -ffp-contract=off -fopenmp (no -ffast-math, no -mfma): the
+ * reference's host code ran on x86-64 with SSE2 double arithmetic, where
+ * every operation rounds separately.
+ *
+ * Every function names the reference lines it restates.  Constants follow
+ * HCGrid/constants.h:57-84 (the PI literal, and macros folded in double).
+ */
+#include "hcg_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* HCGrid/constants.h:57,75-84 */
+#define ORC_MAX_Y    (1073741824ull)
+#define ORC_PI       (3.1415926535897932384626433832)
+#define ORC_TWOTHIRD (2.0 / 3.0)
+#define ORC_HALFPI   (ORC_PI / 2.)
+#define ORC_INV_HALFPI (1. / ORC_HALFPI)
+#define ORC_NORTH_B  (20)
+#define ORC_SOUTH_B  (160)
+#define ORC_DEG2RAD  (ORC_PI / 180.)
+#define ORC_RAD2DEG  (180. / ORC_PI)
+
+/* ------------------------------------------------------------------ */
+/* helpers: HCGrid/helpers.cu                                          */
+/* ------------------------------------------------------------------ */
+
+/* HCGrid/helpers.cu:40-64 */
+uint64_t orc_ilog2(uint64_t arg) {
+    uint64_t r = 0;
+    while (arg > 0xFFFFull) { r += 16; arg >>= 16; }
+    if (arg > 0xFFull) { r |= 8; arg >>= 8; }
+    if (arg > 0xFull)  { r |= 4; arg >>= 4; }
+    if (arg > 0x3ull)  { r |= 2; arg >>= 2; }
+    if (arg > 0x1ull)  { r |= 1; }
+    return r;
+}
+
+/* HCGrid/helpers.cu:67-69 */
+uint64_t orc_isqrt(uint64_t arg) { return (uint64_t)sqrt((double)arg + 0.5); }
+
+/* HCGrid/helpers.cu:83-95 */
+double orc_fmodulo(double v1, double v2) {
+    if (v1 >= 0) {
+        if (v1 < v2) return v1;
+        return fmod(v1, v2);
+    }
+    double tmp = fmod(v1, v2) + v2;
+    if (tmp == v2) return 0.;
+    return tmp;
+}
+
+/* HCGrid/helpers.cu:98-102: always one power of two above floor(log2). */
+uint64_t orc_set_optimal_nside(double target_res) {
+    uint64_t nside = (uint64_t)(sqrt(ORC_PI / 3.) / target_res + 0.5);
+    nside = (uint64_t)1 << (orc_ilog2(nside) + 1);
+    return nside;
+}
+
+/* HCGrid/helpers.cu:105-114 (returns (uint64_t)-1 for non powers of two) */
+uint64_t orc_nside_to_order(uint64_t nside) {
+    if (nside == 0) return ORC_MAX_Y;
+    if (nside & (nside - 1)) return (uint64_t)-1;
+    return orc_ilog2(nside);
+}
+
+/* HCGrid/helpers.cu:128-130 (haversine) */
+double orc_true_angular_distance(double l1, double b1, double l2, double b2) {
+    return 2 * asin(sqrt(pow(sin((b1 - b2) / 2.), 2) +
+                         cos(b1) * cos(b2) * pow(sin((l1 - l2) / 2.), 2)));
+}
+
+/* HCGrid/helpers.cu:133-137 */
+double orc_great_circle_bearing(double l1, double b1, double l2, double b2) {
+    double l_diff_rad = l2 - l1;
+    double cos_b2 = cos(b2);
+    return atan2(cos_b2 * sin(l_diff_rad), cos(b1) * sin(b2) - sin(b1) * cos_b2 * cos(l_diff_rad));
+}
+
+/* ------------------------------------------------------------------ */
+/* HEALPix RING: HCGrid/healpix.cu                                     */
+/* ------------------------------------------------------------------ */
+
+/* HCGrid/healpix.cu:42-65 */
+void orc_hpx_init(orc_hpx *h, uint64_t nside) {
+    h->nside = nside;
+    h->order = orc_nside_to_order(nside);
+    h->nrings = 4 * nside - 1;
+    h->npface = nside * nside;
+    h->ncap = (h->npface - nside) << 1;
+    h->npix = 12 * h->npface;
+    h->fact2 = 4. / (double)h->npix;
+    h->fact1 = (double)(nside << 1) * h->fact2;
+    h->omega = 4. * ORC_PI / (double)h->npix;
+    h->resolution = sqrt(h->omega);
+}
+
+/* HCGrid/healpix.cu:87-105 */
+void orc_ring_info(const orc_hpx *h, uint64_t ring, uint64_t *startpix, uint64_t *npr, int *shifted) {
+    uint64_t nr;
+    if (ring < h->nside) {
+        *shifted = 1;
+        *npr = 4 * ring;
+        *startpix = 2 * ring * (ring - 1);
+    } else if (ring < 3 * h->nside) {
+        *shifted = ((ring - h->nside) & 1) == 0;
+        *npr = 4 * h->nside;
+        *startpix = h->ncap + (ring - h->nside) * (*npr);
+    } else {
+        *shifted = 1;
+        nr = 4 * h->nside - ring;
+        *npr = 4 * nr;
+        *startpix = h->npix - 2 * nr * (nr + 1);
+    }
+}
+
+/* HCGrid/healpix.cu:116-123 */
+uint64_t orc_pix2ring(const orc_hpx *h, uint64_t pix) {
+    if (pix < h->ncap) return (1 + orc_isqrt(1 + 2 * pix)) >> 1;
+    if (pix < h->npix - h->ncap) return (pix - h->ncap) / (4 * h->nside) + h->nside;
+    return 4 * h->nside - ((1 + orc_isqrt(2 * (h->npix - pix) - 1)) >> 1);
+}
+
+/* HCGrid/healpix.cu:166-205 */
+uint64_t orc_loc2pix(const orc_hpx *h, double z, double phi, double sin_theta, int have_sin) {
+    double temp1, temp2, tp, tmp;
+    double za = fabs(z);
+    double tt = orc_fmodulo(phi * ORC_INV_HALFPI, 4.0);
+    uint64_t nl4, jp, jm, ir, kshift, t1, ip;
+    double dn = (double)h->nside;
+
+    if (za <= ORC_TWOTHIRD) {
+        nl4 = 4 * h->nside;
+        temp1 = dn * (0.5 + tt);
+        temp2 = dn * z * 0.75;
+        jp = (uint64_t)(temp1 - temp2);
+        jm = (uint64_t)(temp1 + temp2);
+        ir = h->nside + 1 + jp - jm;
+        kshift = 1 - (ir & 1);
+        t1 = jp + jm + kshift + 1 + nl4 + nl4 - h->nside;
+        if (h->order > 0) ip = (t1 >> 1) & (nl4 - 1);
+        else ip = (t1 >> 1) % nl4;
+        return h->ncap + (ir - 1) * nl4 + ip;
+    }
+    tp = tt - (double)((uint64_t)tt);
+    if ((za < 0.99) || (!have_sin)) tmp = dn * sqrt(3. * (1. - za));
+    else tmp = dn * sin_theta / sqrt((1. + za) / 3.);
+    jp = (uint64_t)(tp * tmp);
+    jm = (uint64_t)((1.0 - tp) * tmp);
+    ir = jp + jm + 1;
+    ip = (uint64_t)(tt * (double)ir);
+    if (z > 0) return 2 * ir * (ir - 1) + ip;
+    return h->npix - 2 * ir * (ir + 1) + ip;
+}
+
+/* HCGrid/healpix.cu:217-224 */
+uint64_t orc_ang2pix(const orc_hpx *h, double theta, double phi) {
+    if (!(theta >= 0 && theta <= ORC_PI)) return ORC_MAX_Y;
+    if ((theta < 0.01) || (theta > ORC_PI - 0.01))
+        return orc_loc2pix(h, cos(theta), phi, sin(theta), 1);
+    return orc_loc2pix(h, cos(theta), phi, 0., 0);
+}
+
+/* HCGrid/healpix.cu:274-320 */
+void orc_pix2loc(const orc_hpx *h, uint64_t pix, double *z, double *phi, double *sin_theta, int *have_sin) {
+    double tmp, fodd;
+    uint64_t ip, itmp, nl4, iphi, iring;
+    *have_sin = 0;
+    if (pix < h->ncap) {
+        iring = (1 + orc_isqrt(1 + 2 * pix)) >> 1;
+        iphi = (pix + 1) - 2 * iring * (iring - 1);
+        tmp = (double)(iring * iring) * h->fact2;
+        *z = 1.0 - tmp;
+        if (*z > 0.99) { *sin_theta = sqrt(tmp * (2. - tmp)); *have_sin = 1; }
+        *phi = ((double)iphi - 0.5) * ORC_HALFPI / (double)iring;
+    } else if (pix < h->npix - h->ncap) {
+        nl4 = 4 * h->nside;
+        ip = pix - h->ncap;
+        /* order is unsigned in the reference, so the `>= 0` branch is always taken */
+        itmp = ip >> (h->order + 2);
+        iring = itmp + h->nside;
+        iphi = ip - nl4 * itmp + 1;
+        fodd = (((iring + h->nside) & 1) > 0) ? 1.0 : 0.5;
+        *z = (2. * (double)h->nside - (double)iring) * h->fact1;
+        *phi = ((double)iphi - fodd) * ORC_PI * 0.75 * h->fact1;
+    } else {
+        ip = h->npix - pix;
+        iring = (1 + orc_isqrt(2 * ip - 1)) >> 1;
+        iphi = 4 * iring + 1 - (ip - 2 * iring * (iring - 1));
+        tmp = (double)(iring * iring) * h->fact2;
+        *z = tmp - 1.0;
+        if (*z < -0.99) { *sin_theta = sqrt(tmp * (2. - tmp)); *have_sin = 1; }
+        *phi = ((double)iphi - 0.5) * ORC_HALFPI / (double)iring;
+    }
+}
+
+/* HCGrid/healpix.cu:336-348 */
+void orc_pix2ang(const orc_hpx *h, uint64_t pix, double *theta, double *phi) {
+    double z, s = 0.;
+    int have;
+    orc_pix2loc(h, pix, &z, phi, &s, &have);
+    *theta = have ? atan2(s, z) : acos(z);
+}
+
+void orc_ang2pix_many(const orc_hpx *h, int64_t n, const double *theta, const double *phi, uint64_t *out) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) out[i] = orc_ang2pix(h, theta[i], phi[i]);
+}
+
+void orc_pix2ang_many(const orc_hpx *h, int64_t n, const uint64_t *pix, double *theta, double *phi) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) orc_pix2ang(h, pix[i], &theta[i], &phi[i]);
+}
+
+void orc_pix2ring_many(const orc_hpx *h, int64_t n, const uint64_t *pix, uint64_t *out) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) out[i] = orc_pix2ring(h, pix[i]);
+}
+
+void orc_ring_info_many(const orc_hpx *h, int64_t n, const uint64_t *ring, uint64_t *startpix, uint64_t *npr) {
+    for (int64_t i = 0; i < n; ++i) {
+        int sh;
+        orc_ring_info(h, ring[i], &startpix[i], &npr[i], &sh);
+    }
+}
+
+/* HCGrid/gridding.cu:77-82: theta = HALFPI - DEG2RAD*lat, phi = DEG2RAD*lon */
+void orc_sample_keys(const orc_hpx *h, int64_t n, const double *lon, const double *lat, uint64_t *keys) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) {
+        double theta = ORC_HALFPI - ORC_DEG2RAD * lat[i];
+        double phi = ORC_DEG2RAD * lon[i];
+        keys[i] = orc_ang2pix(h, theta, phi);
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* pre-sort and the ring lookup table: HCGrid/gridding.cu:70-147       */
+/* ------------------------------------------------------------------ */
+
+/* HCGrid/gridding.cu:260-275: largest i in [left, right) with v[i] < key,
+ * else `right`.  Unsigned 32-bit arithmetic as in the reference. */
+uint32_t orc_search_last_less(const uint64_t *values, uint32_t left, uint32_t right, uint64_t key) {
+    if (right <= left) return right;
+    uint32_t low = left, mid, up = right - 1;
+    while (low < up) {
+        mid = low + ((up - low + 1) >> 1);
+        if (values[mid] < key) low = mid;
+        else up = mid - 1;
+    }
+    if (values[low] < key) return low;
+    return right;
+}
+
+/* Stable LSD radix sort of (key, index) records: the (key, original index)
+ * order of HCGrid/gridding.cu:86-92 with sort_param 2 (parallel_stable_sort). */
+void orc_presort(const orc_hpx *h, uint32_t n, const double *lon, const double *lat,
+                 uint64_t *keys_sorted, uint32_t *perm) {
+    uint64_t *ka = (uint64_t *)malloc(sizeof(uint64_t) * (n ? n : 1));
+    uint64_t *kb = (uint64_t *)malloc(sizeof(uint64_t) * (n ? n : 1));
+    uint32_t *va = (uint32_t *)malloc(sizeof(uint32_t) * (n ? n : 1));
+    uint32_t *vb = (uint32_t *)malloc(sizeof(uint32_t) * (n ? n : 1));
+    orc_sample_keys(h, n, lon, lat, ka);
+    for (uint32_t i = 0; i < n; ++i) va[i] = i;
+    for (int pass = 0; pass < 8; ++pass) {
+        int shift = 8 * pass;
+        uint64_t cnt[257];
+        memset(cnt, 0, sizeof(cnt));
+        for (uint32_t i = 0; i < n; ++i) cnt[((ka[i] >> shift) & 0xFF) + 1]++;
+        int trivial = 0;
+        for (int d = 0; d < 256; ++d) if (cnt[d + 1] == n) trivial = 1;
+        if (trivial) continue;
+        for (int d = 0; d < 256; ++d) cnt[d + 1] += cnt[d];
+        for (uint32_t i = 0; i < n; ++i) {
+            uint64_t pos = cnt[(ka[i] >> shift) & 0xFF]++;
+            kb[pos] = ka[i];
+            vb[pos] = va[i];
+        }
+        uint64_t *tk = ka; ka = kb; kb = tk;
+        uint32_t *tv = va; va = vb; vb = tv;
+    }
+    for (uint32_t i = 0; i < n; ++i) { keys_sorted[i] = ka[i]; perm[i] = va[i]; }
+    keys_sorted[n] = h->npix; /* HCGrid/gridding.cu:100 */
+    free(ka); free(kb); free(va); free(vb);
+}
+
+/* HCGrid/gridding.cu:101-114 */
+void orc_gather(uint32_t n, const uint32_t *perm, const double *src, double *dst) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)n; ++i) dst[i] = src[perm[i]];
+}
+
+/* HCGrid/gridding.cu:116-136 */
+uint32_t orc_ring_lut(const orc_hpx *h, uint32_t n, const uint64_t *keys, uint64_t *firstring,
+                      uint32_t *lut, uint32_t lut_cap) {
+    if (n == 0) return 0;
+    uint64_t first_ring = orc_pix2ring(h, keys[0]);
+    uint32_t temp_count = (uint32_t)(1 + orc_pix2ring(h, keys[n - 1]) - first_ring);
+    if ((uint64_t)temp_count + 1 > lut_cap) return 0;
+    *firstring = first_ring;
+    lut[0] = 0;
+    uint32_t ring_idx = 0;
+    for (uint64_t cnt_ring = 1; cnt_ring < temp_count; ++cnt_ring) {
+        uint64_t startpix, npr;
+        int shifted;
+        orc_ring_info(h, cnt_ring + first_ring, &startpix, &npr, &shifted);
+        uint32_t cnt_ring_idx = orc_search_last_less(keys, ring_idx, n, startpix);
+        if (cnt_ring_idx == n) cnt_ring_idx = ring_idx - 1;
+        ring_idx = cnt_ring_idx + 1;
+        lut[cnt_ring] = ring_idx;
+    }
+    lut[temp_count] = n;
+    return temp_count;
+}
+
+/* ------------------------------------------------------------------ */
+/* gridding: HCGrid/gridding.cu:230-403                                */
+/* ------------------------------------------------------------------ */
+
+/* HCGrid/gridding.cu:230-235 */
+static double orc_sinc(double x) {
+    if (fabs(x) < 1.e-10) return 1.;
+    return sin(x) / x;
+}
+
+/* HCGrid/gridding.cu:238-257.  TAPERED_SINC keeps the reference's
+ * exp(pow(-(arg/p1), 2)) = exp(+(arg/p1)^2) as written. */
+double orc_kernel_weight(const orc_grid *g, double distance, double bearing) {
+    const double *p = g->params;
+    if (g->kernel_type == 1) return exp(-distance * distance * p[0]);
+    if (g->kernel_type == 2) {
+        double ellarg = (pow(p[0], 2.0) * pow(sin(bearing - p[2]), 2.0) +
+                         pow(p[1], 2.0) * pow(cos(bearing - p[2]), 2.0));
+        double Earg = pow(distance / p[0] / p[1], 2.0) / 2. * ellarg;
+        return exp(-Earg);
+    }
+    if (g->kernel_type == 3) {
+        double arg = ORC_PI * distance / p[0];
+        return orc_sinc(arg / p[2]) * exp(pow(-(arg / p[1]), 2.0));
+    }
+    return 0.;
+}
+
+/* Per-group window state, HCGrid/gridding.cu:307-323 */
+typedef struct {
+    uint64_t uring, dring;
+    double l1a, b1a, disc_theta;
+    int polar;
+} orc_window;
+
+static void orc_window_of(const orc_hpx *h, const orc_grid *g, uint64_t firstring, uint32_t usedrings,
+                          double xa, double ya, orc_window *w) {
+    w->l1a = xa * ORC_DEG2RAD;
+    w->b1a = ya * ORC_DEG2RAD;
+    double disc_theta = ORC_HALFPI - w->b1a;
+    double disc_phi = w->l1a;
+    double utheta = disc_theta - g->disc_size;
+    if (utheta * ORC_RAD2DEG < 0) utheta = 0;
+    uint64_t upix = orc_ang2pix(h, utheta, disc_phi);
+    uint64_t uring = orc_pix2ring(h, upix);
+    if (uring < firstring) uring = firstring;
+    utheta = disc_theta + g->disc_size;
+    upix = orc_ang2pix(h, utheta, disc_phi);
+    uint64_t dring = orc_pix2ring(h, upix);
+    if (dring >= firstring + usedrings) dring = firstring + usedrings - 1;
+    w->uring = uring;
+    w->dring = dring;
+    w->disc_theta = disc_theta;
+    /* HCGrid/gridding.cu:341, 352 */
+    w->polar = (disc_theta * ORC_RAD2DEG <= ORC_NORTH_B || disc_theta * ORC_RAD2DEG >= ORC_SOUTH_B);
+}
+
+/* lpix / rpix of ring `ring` for a window, HCGrid/gridding.cu:329-360.
+ * Returns 0 where the reference would `continue` without advancing the ring
+ * (an endless loop there); callers skip the ring. */
+static int orc_ring_window(const orc_hpx *h, const orc_grid *g, const orc_window *w, uint64_t ring,
+                           uint64_t *lpix, uint64_t *rpix) {
+    uint64_t startpix, npr;
+    int shifted;
+    orc_ring_info(h, ring, &startpix, &npr, &shifted);
+    double utheta, uphi;
+    orc_pix2ang(h, startpix, &utheta, &uphi);
+    double disc_phi = w->l1a;
+    uphi = disc_phi - g->disc_size;
+    uint64_t lp = orc_ang2pix(h, utheta, uphi);
+    if (w->polar) lp = startpix;
+    if (!(lp >= startpix && lp < startpix + npr)) return 0;
+    uphi = disc_phi + g->disc_size;
+    uint64_t rp = orc_ang2pix(h, utheta, uphi);
+    if (w->polar) rp = startpix + npr - 1;
+    if (!(rp >= startpix && rp < startpix + npr)) return 0;
+    *lpix = lp;
+    *rpix = rp;
+    return 1;
+}
+
+int64_t orc_grid_compat(const orc_hpx *h, const orc_grid *g, uint32_t n,
+                        const double *lon, const double *lat, const double *data,
+                        const double *weights, const uint64_t *keys,
+                        uint64_t firstring, uint32_t usedrings, const uint32_t *lut,
+                        const double *xwcs, const double *ywcs,
+                        double *datacube, double *weightscube, int nthreads) {
+    return orc_grid_compat_rows(h, g, n, lon, lat, data, weights, keys, firstring, usedrings, lut,
+                                xwcs, ywcs, datacube, weightscube, 0, g->side, nthreads);
+}
+
+int64_t orc_grid_compat_rows(const orc_hpx *h, const orc_grid *g, uint32_t n,
+                             const double *lon, const double *lat, const double *data,
+                             const double *weights, const uint64_t *keys,
+                             uint64_t firstring, uint32_t usedrings, const uint32_t *lut,
+                             const double *xwcs, const double *ywcs,
+                             double *datacube, double *weightscube,
+                             uint32_t row_begin, uint32_t row_end, int nthreads) {
+    const uint32_t side = g->side;
+    const uint32_t factor = g->factor ? g->factor : 1;
+    const int64_t groups_per_row = (side + factor - 1) / factor;
+    if (row_end > side) row_end = side;
+    const int64_t g0 = groups_per_row * (int64_t)row_begin;
+    const int64_t ngroups = groups_per_row * (int64_t)row_end;
+    int64_t hangs = 0;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+    if (n == 0 || usedrings == 0) return 0;
+
+#pragma omp parallel for schedule(dynamic, 8) reduction(+ : hangs)
+    for (int64_t gi = g0; gi < ngroups; ++gi) {
+        /* HCGrid/gridding.cu:287-304: a group is `factor` consecutive cells of a row */
+        uint32_t row = (uint32_t)(gi / groups_per_row);
+        uint32_t left = (uint32_t)(gi % groups_per_row) * factor;
+        uint32_t right = left + factor - 1;
+        if (right >= side) right = side - 1;
+        uint64_t base = (uint64_t)row * side;
+        double tw[64], td[64], l1[64], b1[64];
+        uint32_t m = right - left + 1;
+        for (uint32_t k = 0; k < m; ++k) {
+            uint64_t c = base + left + k;
+            tw[k] = weightscube[c];
+            td[k] = datacube[c];
+            l1[k] = xwcs[c] * ORC_DEG2RAD;
+            b1[k] = ywcs[c] * ORC_DEG2RAD;
+        }
+        orc_window w;
+        orc_window_of(h, g, firstring, usedrings, xwcs[base + left], ywcs[base + left], &w);
+        uint64_t uring = w.uring;
+        /* HCGrid/gridding.cu:326 */
+        uint32_t start_int = lut[uring - firstring];
+        while (uring <= w.dring) {
+            uint32_t end_int = lut[uring - firstring + 1];
+            uint64_t lpix, rpix;
+            if (!orc_ring_window(h, g, &w, uring, &lpix, &rpix)) {
+                hangs++;
+                start_int = end_int;
+                ++uring;
+                continue;
+            }
+            /* HCGrid/gridding.cu:363-367: start_int-1 wraps to 0xFFFFFFFF on
+             * the first used ring, which then contributes nothing. */
+            uint32_t upix_idx = orc_search_last_less(keys, start_int - 1u, end_int, lpix);
+            ++upix_idx;
+            if (upix_idx > end_int) upix_idx = n;
+            while (upix_idx < n) {
+                double l2 = lon[upix_idx] * ORC_DEG2RAD;
+                double b2 = lat[upix_idx] * ORC_DEG2RAD;
+                /* HCGrid/gridding.cu:373: the kernel recomputes the sample's pixel */
+                uint64_t spix = orc_ang2pix(h, ORC_HALFPI - b2, l2);
+                if (spix > rpix) break;
+                double in_w = weights ? weights[upix_idx] : 1.0;
+                double in_d = data[upix_idx];
+                for (uint32_t k = 0; k < m; ++k) {
+                    double sdist = orc_true_angular_distance(l1[k], b1[k], l2, b2) * ORC_RAD2DEG;
+                    double sbear = 0.;
+                    if (g->bearing_needed) sbear = orc_great_circle_bearing(l1[k], b1[k], l2, b2);
+                    if (sdist < g->sphere_radius) {
+                        double sweight = orc_kernel_weight(g, sdist, sbear);
+                        double tweight = in_w * sweight;
+                        td[k] += in_d * tweight;
+                        tw[k] += tweight;
+                    }
+                }
+                ++upix_idx;
+            }
+            start_int = end_int;
+            ++uring;
+        }
+        for (uint32_t k = 0; k < m; ++k) {
+            uint64_t c = base + left + k;
+            datacube[c] = td[k];
+            weightscube[c] = tw[k];
+        }
+    }
+    return hangs;
+}
+
+void orc_grid_bruteforce(const orc_hpx *h, const orc_grid *g, uint32_t n,
+                         const double *lon, const double *lat, const double *data,
+                         const double *weights, const uint64_t *keys,
+                         uint64_t firstring, uint32_t usedrings,
+                         const double *xwcs, const double *ywcs,
+                         double *datacube, double *weightscube, int nthreads) {
+    const uint32_t side = g->side;
+    const uint32_t factor = g->factor ? g->factor : 1;
+    const int64_t ncells = (int64_t)side * side;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+    if (n == 0 || usedrings == 0) return;
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int64_t c = 0; c < ncells; ++c) {
+        uint32_t row = (uint32_t)(c / side), col = (uint32_t)(c % side);
+        uint64_t anchor = (uint64_t)row * side + (col / factor) * factor;
+        orc_window w;
+        orc_window_of(h, g, firstring, usedrings, xwcs[anchor], ywcs[anchor], &w);
+        double l1 = xwcs[c] * ORC_DEG2RAD, b1 = ywcs[c] * ORC_DEG2RAD;
+        double td = datacube[c], tw = weightscube[c];
+        uint64_t cached_ring = (uint64_t)-1, lp = 0, rp = 0;
+        int ok = 0;
+        for (uint32_t s = 0; s < n; ++s) {
+            uint64_t r = orc_pix2ring(h, keys[s]);
+            if (r < w.uring || r > w.dring) continue;
+            if (r == firstring) continue; /* start_int - 1 underflow */
+            if (r != cached_ring) {
+                cached_ring = r;
+                ok = orc_ring_window(h, g, &w, r, &lp, &rp);
+            }
+            if (!ok) continue;
+            if (keys[s] < lp || keys[s] > rp) continue;
+            double l2 = lon[s] * ORC_DEG2RAD, b2 = lat[s] * ORC_DEG2RAD;
+            double sdist = orc_true_angular_distance(l1, b1, l2, b2) * ORC_RAD2DEG;
+            if (!(sdist < g->sphere_radius)) continue;
+            double sbear = g->bearing_needed ? orc_great_circle_bearing(l1, b1, l2, b2) : 0.;
+            double tweight = (weights ? weights[s] : 1.0) * orc_kernel_weight(g, sdist, sbear);
+            td += data[s] * tweight;
+            tw += tweight;
+        }
+        datacube[c] = td;
+        weightscube[c] = tw;
+    }
+}
+
+void orc_grid_disc_textbook(uint32_t ncells, uint32_t n, const double *lon, const double *lat,
+                            const double *data, const double *xwcs, const double *ywcs,
+                            double sphere_radius_deg, double p0,
+                            double *datacube, double *weightscube, int nthreads) {
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+    const long double d2r = 3.14159265358979323846264338327950288L / 180.0L;
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int64_t c = 0; c < (int64_t)ncells; ++c) {
+        long double a1 = xwcs[c] * d2r, d1 = ywcs[c] * d2r;
+        long double px = cosl(d1) * cosl(a1), py = cosl(d1) * sinl(a1), pz = sinl(d1);
+        long double sv = 0.0L, sw = 0.0L;
+        for (uint32_t s = 0; s < n; ++s) {
+            long double a2 = lon[s] * d2r, d2 = lat[s] * d2r;
+            long double qx = cosl(d2) * cosl(a2), qy = cosl(d2) * sinl(a2), qz = sinl(d2);
+            long double cx = py * qz - pz * qy, cy = pz * qx - px * qz, cz = px * qy - py * qx;
+            long double dist = atan2l(sqrtl(cx * cx + cy * cy + cz * cz), px * qx + py * qy + pz * qz) / d2r;
+            if (dist < (long double)sphere_radius_deg) {
+                long double wgt = expl(-dist * dist * (long double)p0);
+                sv += (long double)data[s] * wgt;
+                sw += wgt;
+            }
+        }
+        datacube[c] = (double)sv;
+        weightscube[c] = (double)sw;
+    }
+}
+
+/* HCGrid/gmap.cpp:305-312 (NaN where the weight sum is 0, by IEEE 0/0) */
+void orc_normalize(int64_t ncells, const double *datacube, const double *weightscube, double *map) {
+    for (int64_t i = 0; i < ncells; ++i) map[i] = datacube[i] / weightscube[i];
+}
+
+/* SIN deprojection + zenithal rotation in long double (Calabretta & Greisen
+ * 2002, eqs. 2, 54-55 with PV2_1 = PV2_2 = 0, and the native-to-celestial
+ * rotation, eq. 2 of the same paper, with phi_p = LONPOLE = 180 deg). */
+void orc_wcs_sin(uint32_t naxis1, uint32_t naxis2, double crpix1, double crpix2,
+                 double cdelt1, double cdelt2, double crval1, double crval2,
+                 double *xwcs, double *ywcs) {
+    const long double pi = 3.14159265358979323846264338327950288L;
+    const long double d2r = pi / 180.0L;
+    long double a0 = crval1 * d2r, dl0 = crval2 * d2r;
+    for (uint32_t yy = 0; yy < naxis2; ++yy) {
+        for (uint32_t xx = 0; xx < naxis1; ++xx) {
+            long double x = (long double)cdelt1 * ((long double)(xx + 1) - crpix1) * d2r;
+            long double y = (long double)cdelt2 * ((long double)(yy + 1) - crpix2) * d2r;
+            long double r = sqrtl(x * x + y * y);
+            long double phi = atan2l(x, -y);        /* native longitude */
+            long double theta = acosl(r);          /* native latitude */
+            long double dphi = phi - pi;           /* phi - phi_p */
+            long double sd = sinl(theta) * sinl(dl0) + cosl(theta) * cosl(dl0) * cosl(dphi);
+            long double dec = asinl(sd);
+            long double ra = a0 + atan2l(-cosl(theta) * sinl(dphi),
+                                         sinl(theta) * cosl(dl0) - cosl(theta) * sinl(dl0) * cosl(dphi));
+            long double ra_deg = ra / d2r;
+            while (ra_deg < 0) ra_deg += 360.0L;
+            while (ra_deg >= 360.0L) ra_deg -= 360.0L;
+            uint64_t c = (uint64_t)yy * naxis1 + xx;
+            xwcs[c] = (double)ra_deg;
+            ywcs[c] = (double)(dec / d2r);
+        }
+    }
+}
