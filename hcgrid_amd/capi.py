@@ -81,6 +81,10 @@ PROTOTYPES = {
     "hcg_set_optimal_nside": (_u64, [_dbl]),
     "hcg_prepare_grid_kernel": (_int, [_u32, C.POINTER(_dbl), _dbl, _dbl, C.POINTER(Kernel),
                                        C.POINTER(Healpix)]),
+    "hcg_host_ang2pix": (_u64, [C.POINTER(Healpix), _dbl, _dbl]),
+    "hcg_host_pix2ring": (_u64, [C.POINTER(Healpix), _u64]),
+    "hcg_host_ring_info": (None, [C.POINTER(Healpix), _u64, C.POINTER(_u64), C.POINTER(_u64)]),
+    "hcg_host_pix2theta": (_dbl, [C.POINTER(Healpix), _u64]),
     "hcg_presort_workspace_size": (_sz, [_u32]),
     "hcg_presort": (_int, [C.POINTER(Healpix), _u32, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
     "hcg_gather_samples": (_int, [_u32, _vp, _vp, _vp, _vp, _int, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),

@@ -156,6 +156,21 @@ extern "C" int hcg_prepare_grid_kernel(uint32_t kernel_type, const double* kerne
   return HCG_OK;
 }
 
+extern "C" uint64_t hcg_host_ang2pix(const hcg_healpix* hp, double theta, double phi) {
+  return hp ? ang2pix(*hp, theta, phi) : kMaxY;
+}
+
+extern "C" uint64_t hcg_host_pix2ring(const hcg_healpix* hp, uint64_t pix) { return hp ? pix2ring(*hp, pix) : 0; }
+
+extern "C" void hcg_host_ring_info(const hcg_healpix* hp, uint64_t ring, uint64_t* startpix, uint64_t* npr) {
+  uint64_t s = 0, n = 0;
+  if (hp) ring_info(*hp, ring, s, n);
+  if (startpix) *startpix = s;
+  if (npr) *npr = n;
+}
+
+extern "C" double hcg_host_pix2theta(const hcg_healpix* hp, uint64_t pix) { return hp ? pix2theta(*hp, pix) : 0.0; }
+
 // SIN deprojection (Calabretta & Greisen 2002, Paper II sec. 5.1.5, PV2_1 =
 // PV2_2 = 0) and the native -> celestial rotation (Paper II eq. 2) with
 // (alpha_p, delta_p) = CRVAL and phi_p = LONPOLE.  Cell (x, y) <- pixel

@@ -201,13 +201,223 @@ __global__ void __launch_bounds__(256) k_grid_cells(GridParams P, GridBuffers B)
   B.wc[c] = acc_w;
 }
 
+// ---------------------------------------------------------------------------
+// Wave-tiled kernel: one 64-lane workgroup per 16 x 8 tile of cells.  Lane l
+// owns column (l & 15) and two vertically adjacent cells in rows
+// 2*(l >> 4) + {0, 1}.  The wave walks the union of its cells' ring ranges in
+// ascending order; per non-empty ring it
+//   1. evaluates every cell's [lpix, rpix] window (per-ring cos/sin table),
+//   2. finds the sorted-index range of the union window with a 64-way
+//      cooperative search of the keys (3 coalesced probes instead of ~15
+//      dependent loads),
+//   3. stages that range (records as x/y/z/v columns + keys) in LDS with
+//      coalesced loads, chunk by chunk,
+//   4. lets every lane locate its cells' sub-ranges by binary search in LDS
+//      and accumulate over its union sub-range, two cells per record read.
+// Cells in a column share their longitude window, so lanes of one column
+// read the same LDS words (broadcast) and columns are ~11 samples apart,
+// which spreads the 16 distinct addresses over the banks.
+// ---------------------------------------------------------------------------
+constexpr int kTileCols = 16;
+constexpr int kTileRows = 8;
+constexpr int kCellsPerLane = 2;
+constexpr uint32_t kStageCap = 384;
+
+// 64-way search over keys[lo, hi): first index whose key >= K (UPPER=false) or
+// > K (UPPER=true).  Wave-uniform inputs and result.
+template <bool UPPER>
+__device__ inline uint32_t wave_search(const uint32_t* __restrict__ keys, uint32_t lo, uint32_t hi, uint32_t K) {
+  const uint32_t lane = __lane_id();
+  while (hi - lo > 64) {
+    const uint32_t step = (hi - lo + 63) / 64;
+    const uint32_t p = lo + lane * step;
+    const bool inr = p < hi;
+    const uint32_t kv = inr ? keys[p] : 0xFFFFFFFFu;
+    const bool ge = inr ? (UPPER ? kv > K : kv >= K) : true;
+    const uint64_t m = __ballot(ge);
+    if (m == 0) {  // all 64 probes in range and below K: the answer is past the last probe
+      lo = lo + 63 * step + 1;
+      continue;
+    }
+    const uint32_t f = (uint32_t)__ffsll((unsigned long long)m) - 1;
+    const uint32_t pf = lo + f * step;
+    if (f == 0) return lo;
+    const uint32_t nlo = lo + (f - 1) * step + 1;
+    hi = pf < hi ? pf : hi;
+    lo = nlo;
+  }
+  const uint32_t p = lo + lane;
+  const bool inr = p < hi;
+  const uint32_t kv = inr ? keys[p] : 0u;
+  const bool ge = inr && (UPPER ? kv > K : kv >= K);
+  const uint64_t m = __ballot(ge);
+  return m ? lo + (uint32_t)__ffsll((unsigned long long)m) - 1 : hi;
+}
+
+__device__ inline uint32_t wave_min_u32(uint32_t v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const uint32_t o = __shfl_xor(v, off, 64);
+    v = o < v ? o : v;
+  }
+  return v;
+}
+
+__device__ inline uint32_t wave_max_u32(uint32_t v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const uint32_t o = __shfl_xor(v, off, 64);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+
+template <int KT, bool HAS_W>
+__global__ void __launch_bounds__(64) k_grid_tiles(GridParams P, GridBuffers B, uint32_t tiles_x,
+                                                   uint32_t tiles_y) {
+  __shared__ double s_x[kStageCap], s_y[kStageCap], s_z[kStageCap], s_v[kStageCap];
+  __shared__ uint32_t s_key[kStageCap];
+
+  const hcg_ring_meta m = *B.meta;
+  if (m.usedrings == 0) return;
+  // XCD-aware order: consecutive tiles land on the same XCD (workgroups are
+  // dealt round-robin over the 8 XCDs), so neighbouring tiles share L2.
+  const uint32_t ntiles = tiles_x * tiles_y;
+  uint32_t tile = blockIdx.x;
+  {
+    const uint32_t per = (ntiles + 7) / 8;
+    const uint32_t t2 = (blockIdx.x % 8) * per + blockIdx.x / 8;
+    if ((ntiles % 8) == 0) tile = t2;
+  }
+  const uint32_t lane = __lane_id();
+  const uint32_t col = (tile % tiles_x) * kTileCols + (lane & 15);
+  const uint32_t row0 = (tile / tiles_x) * kTileRows + (lane >> 4) * kCellsPerLane;
+
+  // per-cell state
+  bool live[kCellsPerLane];
+  uint64_t cell[kCellsPerLane];
+  double px[kCellsPerLane], py[kCellsPerLane], pz[kCellsPerLane], l1[kCellsPerLane], b1[kCellsPerLane];
+  double acc_d[kCellsPerLane], acc_w[kCellsPerLane];
+  Window win[kCellsPerLane];
+  uint32_t lane_ur = 0xFFFFFFFFu, lane_dr = 0;
+#pragma unroll
+  for (int k = 0; k < kCellsPerLane; ++k) {
+    const uint32_t row = row0 + k;
+    live[k] = (col < P.side) && (row < P.side);
+    const uint32_t rr = live[k] ? row : 0, cc = live[k] ? col : 0;
+    cell[k] = (uint64_t)rr * P.side + cc;
+    const uint64_t anchor = (uint64_t)rr * P.side + (cc / P.factor) * P.factor;
+    win[k] = window_of(P, m.firstring, m.usedrings, B.xwcs[anchor], B.ywcs[anchor]);
+    l1[k] = B.xwcs[cell[k]] * kDeg2Rad;
+    b1[k] = B.ywcs[cell[k]] * kDeg2Rad;
+    double sb, cb, sl, cl;
+    sincos(b1[k], &sb, &cb);
+    sincos(l1[k], &sl, &cl);
+    px[k] = cb * cl;
+    py[k] = cb * sl;
+    pz[k] = sb;
+    acc_d[k] = live[k] ? B.dc[cell[k]] : 0.0;
+    acc_w[k] = live[k] ? B.wc[cell[k]] : 0.0;
+    if (live[k] && win[k].uring <= win[k].dring) {
+      lane_ur = min(lane_ur, (uint32_t)win[k].uring);
+      lane_dr = max(lane_dr, (uint32_t)win[k].dring);
+    }
+  }
+  uint32_t r_begin = wave_min_u32(lane_ur);
+  const uint32_t r_end = wave_max_u32(lane_dr);
+  if (r_begin == m.firstring) ++r_begin;  // the first used ring contributes nothing
+
+  for (uint32_t r = r_begin; r <= r_end && r_begin <= r_end; ++r) {
+    const uint32_t kr = r - m.firstring;
+    const uint32_t start = B.lut[kr], end = B.lut[kr + 1];
+    if (start == end) continue;
+    const double rz = B.ring_z[kr], rs = B.ring_s[kr];
+    uint32_t lp[kCellsPerLane], rp[kCellsPerLane];
+    bool act[kCellsPerLane];
+    uint32_t lane_lp = 0xFFFFFFFFu, lane_rp = 0;
+#pragma unroll
+    for (int k = 0; k < kCellsPerLane; ++k) {
+      act[k] = live[k] && r >= win[k].uring && r <= win[k].dring;
+      uint64_t a = 1, b = 0;
+      if (act[k]) act[k] = ring_window(P, win[k], r, rz, rs, a, b) && a <= b;
+      lp[k] = (uint32_t)a;
+      rp[k] = (uint32_t)b;
+      if (act[k]) {
+        lane_lp = min(lane_lp, lp[k]);
+        lane_rp = max(lane_rp, rp[k]);
+      }
+    }
+    const uint32_t LP = wave_min_u32(lane_lp), RP = wave_max_u32(lane_rp);
+    if (LP > RP) continue;
+    const uint32_t lo_u = wave_search<false>(B.keys, start, end, LP);
+    const uint32_t hi_u = wave_search<true>(B.keys, lo_u, end, RP);
+    for (uint32_t cb = lo_u; cb < hi_u; cb += kStageCap) {
+      const uint32_t cnt = min(kStageCap, hi_u - cb);
+      __syncthreads();  // previous chunk fully consumed
+      for (uint32_t j = lane; j < cnt; j += 64) {
+        const hcg_sample q = B.rec[cb + j];
+        s_x[j] = q.x;
+        s_y[j] = q.y;
+        s_z[j] = q.z;
+        s_v[j] = q.v;
+        s_key[j] = B.keys[cb + j];
+      }
+      __syncthreads();
+      uint32_t lo[kCellsPerLane], hi[kCellsPerLane];
+      uint32_t jb = 0xFFFFFFFFu, je = 0;
+#pragma unroll
+      for (int k = 0; k < kCellsPerLane; ++k) {
+        lo[k] = 0;
+        hi[k] = 0;
+        if (act[k]) {
+          lo[k] = lower_bound_u32(s_key, 0, cnt, lp[k]);
+          hi[k] = upper_bound_u32(s_key, lo[k], cnt, rp[k]);
+          if (lo[k] < hi[k]) {
+            jb = min(jb, lo[k]);
+            je = max(je, hi[k]);
+          }
+        }
+      }
+      for (uint32_t j = jb; j < je; ++j) {
+        const double qx = s_x[j], qy = s_y[j], qz = s_z[j], qv = s_v[j];
+#pragma unroll
+        for (int k = 0; k < kCellsPerLane; ++k) {
+          const double dx = px[k] - qx, dy = py[k] - qy, dz = pz[k] - qz;
+          const double h = 0.25 * fma(dx, dx, fma(dy, dy, dz * dz));
+          const bool band = j >= lo[k] && j < hi[k];
+          bool in = band && h < P.h_lo;
+          if (band && !in && h < P.h_hi) {
+            const uint32_t gi = cb + j;
+            in = ref_distance_deg(l1[k], b1[k], B.lon_s[gi] * kDeg2Rad, B.lat_s[gi] * kDeg2Rad) <
+                 P.kern.sphere_radius;
+          }
+          if (in) {
+            double wt = pair_weight<KT>(P, h, l1[k], b1[k], B, cb + j);
+            if constexpr (HAS_W) wt = B.w_s[cb + j] * wt;
+            acc_d[k] = fma(qv, wt, acc_d[k]);
+            acc_w[k] += wt;
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < kCellsPerLane; ++k) {
+    if (live[k]) {
+      B.dc[cell[k]] = acc_d[k];
+      B.wc[cell[k]] = acc_w[k];
+    }
+  }
+}
+
 template <int KT, bool HAS_W>
 int launch_grid(const GridParams& P, const GridBuffers& B, hipStream_t s) {
-  const uint32_t tiles_x = (P.side + 15) / 16;
-  const uint64_t blocks = (uint64_t)tiles_x * tiles_x;
+  const uint32_t tiles_x = (P.side + kTileCols - 1) / kTileCols;
+  const uint32_t tiles_y = (P.side + kTileRows - 1) / kTileRows;
+  const uint64_t blocks = (uint64_t)tiles_x * tiles_y;
   if (blocks == 0) return HCG_OK;
-  k_grid_cells<KT, HAS_W><<<(uint32_t)blocks, 256, 0, s>>>(P, B);
-  HCG_CHECK_LAUNCH("k_grid_cells");
+  k_grid_tiles<KT, HAS_W><<<(uint32_t)blocks, 64, 0, s>>>(P, B, tiles_x, tiles_y);
+  HCG_CHECK_LAUNCH("k_grid_tiles");
   return HCG_OK;
 }
 

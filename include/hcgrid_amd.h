@@ -110,6 +110,13 @@ int hcg_prepare_grid_kernel(uint32_t kernel_type, const double *kernel_params,
                             double sphere_radius_deg, double hpx_max_res_deg,
                             hcg_kernel *kern, hcg_healpix *hp);
 
+/* Host evaluations of the same HEALPix functions the device uses (for
+ * callers that mirror the reference's h_* helpers, HCGrid/healpix.cu:52-348). */
+uint64_t hcg_host_ang2pix(const hcg_healpix *hp, double theta, double phi);
+uint64_t hcg_host_pix2ring(const hcg_healpix *hp, uint64_t pix);
+void     hcg_host_ring_info(const hcg_healpix *hp, uint64_t ring, uint64_t *startpix, uint64_t *npr);
+double   hcg_host_pix2theta(const hcg_healpix *hp, uint64_t pix);
+
 /* ---- the device path ---- */
 
 /* Workspace bytes hcg_presort() needs for n samples. */

@@ -1,0 +1,188 @@
+"""Pins the oracle's pre-sort, ring LUT and gridding restatement (CPU only).
+
+* presort     : equals numpy's stable argsort of the keys.
+* ring LUT    : equals the definition lut[k] = first index with key >=
+                startpix(firstring + k), startpix from the closed form.
+* search      : searchLastPosLessThan semantics incl. the unsigned wrap.
+* grid        : the loop restatement (LUT + binary search + break test)
+                equals a pair-by-pair evaluation of the selection rule, bit
+                for bit; and where the reference's window holds the whole
+                disc (|dec| < 20 deg, first used ring far away) it equals a
+                textbook long-double disc gridder.
+* kernels     : closed-form values of the three weight functions.
+* WCS         : closed forms of the SIN deprojection.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+from hcgrid_amd import synth
+from healpix_indep import ring_start_closed_form
+
+from helpers import map_errors, normalise
+
+
+def test_presort_is_stable_argsort():
+    h = O.hpx(8192)
+    wl = synth.raster_small(50, 400, 10, seed=2)
+    lon, lat = wl.lon.copy(), wl.lat.copy()
+    lon[::7] = lon[3]  # many equal keys
+    lat[::7] = lat[3]
+    keys = O.sample_keys(h, lon, lat)
+    ks, perm = O.presort(h, lon, lat)
+    ref = np.argsort(keys, kind="stable")
+    assert np.array_equal(perm, ref.astype(np.uint32))
+    assert np.array_equal(ks[:-1], keys[ref])
+    assert ks[-1] == h.npix
+
+
+def test_ring_lut_definition():
+    h = O.hpx(8192)
+    wl = synth.raster_250k()
+    ks, perm = O.presort(h, wl.lon, wl.lat)
+    n = wl.n
+    fr, used, lut = O.ring_lut(h, ks, n)
+    assert fr == int(O.pix2ring(h, ks[:1])[0])
+    assert used == int(O.pix2ring(h, ks[n - 1:n])[0]) - fr + 1
+    starts = ring_start_closed_form(8192, np.arange(fr, fr + used))
+    expect = np.searchsorted(ks[:n], starts, side="left")
+    expect[0] = 0
+    assert np.array_equal(lut[:used], expect.astype(np.uint32))
+    assert lut[used] == n
+
+
+def test_search_last_less_semantics():
+    L = O.lib()
+    rng = np.random.default_rng(0)
+    for _ in range(300):
+        v = np.sort(rng.integers(0, 50, size=rng.integers(1, 40))).astype(np.uint64)
+        left = int(rng.integers(0, len(v) + 1))
+        right = int(rng.integers(0, len(v) + 1))
+        key = int(rng.integers(0, 55))
+        got = L.orc_search_last_less(v.ctypes.data_as(C.c_void_p), left, right, key)
+        cand = [i for i in range(left, right) if v[i] < key]
+        expect = cand[-1] if (right > left and v[left] < key) else right
+        if right > left and v[left] < key:
+            expect = max(cand)
+        assert got == expect
+    v = np.arange(10, dtype=np.uint64)
+    # left = start_int - 1 wrapped to 0xFFFFFFFF: returns right (the first-ring skip)
+    assert L.orc_search_last_less(v.ctypes.data_as(C.c_void_p), 0xFFFFFFFF, 5, 3) == 5
+
+
+def _setup(wl, side=None, factor=1, sigma=0.02, radius=0.06, res=0.01, kernel_type=O.GAUSS1D, params=None):
+    t = wl.target
+    side = side or t.side
+    xw, yw = O.wcs_sin(side, side, side / 2.0, side / 2.0, -t.cdelt, t.cdelt, t.crval1, t.crval2)
+    r = O.pipeline(wl.lon, wl.lat, wl.values, xw, yw, side, sigma, radius, res, factor=factor,
+                   kernel_type=kernel_type, kernel_params=params)
+    return r, xw, yw
+
+
+@pytest.mark.parametrize("dec,factor", [(30.0, 1), (30.0, 2), (30.0, 3), (-45.0, 1), (0.0, 1), (72.0, 1),
+                                        (84.0, 1)])
+def test_compat_equals_pairwise_bruteforce(dec, factor):
+    sp = 0.0032 / max(math.cos(math.radians(dec)), 0.1)
+    wl = synth.raster_small(30, 120, 16, dec_c=dec, spacing_ra=sp, seed=31)
+    r, xw, yw = _setup(wl, factor=factor)
+    dc, wc = O.grid_bruteforce(r["h"], r["g"], r["lon_s"], r["lat_s"], r["data_s"], None, r["keys_sorted"],
+                               r["firstring"], r["usedrings"], xw, yw)
+    assert r["hangs"] == 0
+    assert np.array_equal(dc, r["datacube"]) and np.array_equal(wc, r["weightscube"])
+    assert (wc > 0).sum() > 0
+
+
+@pytest.mark.parametrize("kt,params", [(O.GAUSS2D, [0.02, 0.012, 0.4]), (O.TAPERED_SINC, [0.02, 1.55, 2.52])])
+def test_compat_equals_bruteforce_other_kernels(kt, params):
+    wl = synth.raster_small(30, 120, 16, seed=37)
+    r, xw, yw = _setup(wl, kernel_type=kt, params=params)
+    dc, wc = O.grid_bruteforce(r["h"], r["g"], r["lon_s"], r["lat_s"], r["data_s"], None, r["keys_sorted"],
+                               r["firstring"], r["usedrings"], xw, yw)
+    assert np.array_equal(dc, r["datacube"]) and np.array_equal(wc, r["weightscube"])
+
+
+def test_first_used_ring_contributes_nothing():
+    # a single scan line: every sample lies in the first used ring, so the
+    # reference's start_int - 1 wrap leaves every cell empty
+    wl = synth.raster_small(1, 300, 12, seed=3)
+    r, xw, yw = _setup(wl)
+    assert r["usedrings"] == 1
+    assert np.all(r["weightscube"] == 0)
+
+
+@pytest.mark.parametrize("dec", [0.0, -12.0, 15.0])
+def test_compat_equals_textbook_disc_near_equator(dec):
+    # At |dec| < 20 deg the +-disc longitude window (disc = R + resolution)
+    # covers the whole R-disc, and with the first used ring far away the
+    # reference's selection is exactly "every sample within R".
+    wl = synth.raster_small(40, 160, 20, dec_c=dec, seed=41)
+    lon = np.concatenate([[wl.lon[0]], wl.lon])
+    lat = np.concatenate([[dec + 3.0], wl.lat])  # a far sample owns the first used ring
+    val = np.concatenate([[0.0], wl.values.astype(np.float64)])
+    t = wl.target
+    xw, yw = O.wcs_sin(t.side, t.side, t.side / 2.0, t.side / 2.0, -t.cdelt, t.cdelt, t.crval1, t.crval2)
+    r = O.pipeline(lon, lat, val, xw, yw, t.side, 0.02, 0.06, 0.01)
+    p0 = 0.5 / 0.02 ** 2
+    dc, wc = O.grid_disc_textbook(lon, lat, val, xw, yw, 0.06, p0)
+    e = map_errors(normalise(r["datacube"], r["weightscube"]), normalise(dc, wc))
+    assert e["nan_match"]
+    # FP64 vs long-double summation of noise-valued terms: 1e-12 of the peak,
+    # 1e-9 per cell.  One sample more or less would move a weight sum by
+    # >= exp(-4.5) / sum(w) ~ 1e-4, so the 1e-12 weight check pins the selection.
+    assert e["max_rel"] < 1e-12 and e["cell_rel"] < 1e-9
+    assert np.allclose(r["weightscube"], wc, rtol=1e-12, atol=0)
+
+
+def test_compat_window_truncates_east_west_at_high_dec():
+    # At dec 60 the window is +-disc in longitude without 1/cos(dec), so it
+    # holds only part of the disc: compat weights are strictly below the disc's.
+    wl = synth.raster_small(40, 160, 12, dec_c=60.0, spacing_ra=0.0064, seed=43)
+    r, xw, yw = _setup(wl)
+    dc, wc = O.grid_disc_textbook(wl.lon, wl.lat, wl.values, xw, yw, 0.06, 0.5 / 0.02 ** 2)
+    inner = r["weightscube"] > 0
+    assert np.all(r["weightscube"][inner] < wc[inner])
+
+
+def test_kernel_weight_closed_forms():
+    L = O.lib()
+    sig = 0.02
+    g1 = O.grid_params(O.GAUSS1D, [0.5 / sig ** 2], 0.06, 0.001, 4)
+    assert L.orc_kernel_weight(C.byref(g1), 0.0, 0.0) == 1.0
+    assert math.isclose(L.orc_kernel_weight(C.byref(g1), sig, 0.0), math.exp(-0.5), rel_tol=1e-15)
+    # circular GAUSS2D (sigma_maj = sigma_min) is exp(-d^2 / (2 sigma^2)) for any bearing
+    g2 = O.grid_params(O.GAUSS2D, [sig, sig, 0.3], 0.06, 0.001, 4)
+    for d, b in ((0.01, 0.0), (0.03, 1.1), (0.05, -2.0)):
+        assert math.isclose(L.orc_kernel_weight(C.byref(g2), d, b), math.exp(-d * d / (2 * sig * sig)),
+                            rel_tol=1e-13)
+    g3 = O.grid_params(O.TAPERED_SINC, [sig, 1.55, 2.52], 0.06, 0.001, 4)
+    assert L.orc_kernel_weight(C.byref(g3), 0.0, 0.0) == 1.0
+    d = 0.01
+    arg = math.pi * d / sig
+    expect = math.sin(arg / 2.52) / (arg / 2.52) * math.exp((arg / 1.55) ** 2)  # the reference's sign
+    assert math.isclose(L.orc_kernel_weight(C.byref(g3), d, 0.0), expect, rel_tol=1e-13)
+
+
+def test_wcs_sin_closed_forms():
+    side, c = 101, 0.05
+    xw, yw = O.wcs_sin(side, side, 51.0, 51.0, -c, c, 123.0, 41.0)
+    X = np.arange(side)
+    # reference pixel (x+1 = CRPIX) maps to CRVAL
+    k0 = 50 * side + 50
+    assert abs(xw[k0] - 123.0) < 1e-12 and abs(yw[k0] - 41.0) < 1e-12
+    # central column: dec = dec0 + asin(y), ra = ra0
+    col = X * side + 50
+    y = c * (X + 1 - 51.0)
+    assert np.allclose(yw[col], 41.0 + np.degrees(np.arcsin(np.radians(y))), atol=1e-11)
+    assert np.allclose(xw[col], 123.0, atol=1e-11)
+    # forward orthographic projection of every cell returns its pixel offsets
+    a, d = np.radians(xw - 123.0), np.radians(yw)
+    d0 = math.radians(41.0)
+    fx = np.degrees(np.cos(d) * np.sin(a))
+    fy = np.degrees(np.sin(d) * math.cos(d0) - np.cos(d) * math.sin(d0) * np.cos(a))
+    px = (np.arange(side * side) % side) + 1
+    py = (np.arange(side * side) // side) + 1
+    assert np.allclose(-fx, c * (px - 51.0), atol=1e-11)  # CDELT1 < 0
+    assert np.allclose(fy, c * (py - 51.0), atol=1e-11)
