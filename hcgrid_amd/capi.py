@@ -91,7 +91,12 @@ PROTOTYPES = {
     "hcg_ring_lut": (_int, [C.POINTER(Healpix), _u32, _vp, _vp, _vp, _vp, _vp, _vp]),
     "hcg_grid_plain": (_int, [C.POINTER(Healpix), C.POINTER(Kernel), _u32, _u32, _vp, _vp, _vp, _vp, _vp,
                               _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "hcg_channels_padded_len": (_u32, [_u32]),
+    "hcg_gather_channels": (_int, [_u32, _u32, _vp, _vp, _int, _int, _vp, _vp, _vp]),
+    "hcg_grid_mfma": (_int, [C.POINTER(Healpix), C.POINTER(Kernel), _u32, _u32, _u32, _int, _vp, _vp, _vp, _vp,
+                             _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _int, _vp]),
     "hcg_normalize": (_int, [_u64, _vp, _vp, _vp, _vp]),
+    "hcg_eval_exp_neg": (_int, [_u32, _vp, _vp, _vp]),
     "hcg_wcs_sin_p2s": (_int, [_u32, _u32, _dbl, _dbl, _dbl, _dbl, _dbl, _dbl, _dbl, _vp, _vp]),
 }
 

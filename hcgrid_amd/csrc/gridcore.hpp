@@ -1,0 +1,227 @@
+// gridcore.hpp -- device pieces shared by the gridding kernels: the
+// reference's window/selection logic (HCGrid/gridding.cu:306-360), the
+// distance and weight evaluation, and wave-level search/reduction helpers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "common.hpp"
+#include "expneg.hpp"
+#include "healpix.hpp"
+
+namespace hcg {
+
+struct GridParams {
+  hcg_healpix hp;
+  double disc_size;
+  double sphere_radius;
+  double params[3];
+  double c_lo, c_hi;   // guard band around the radius threshold on c2 = |p - q|^2 = 4 sin^2(d/2)
+  // GAUSS1D exponent in units of ln2/64: y = c2 (k0 + c2 (k1 + c2 (k2 + c2 k3))) for c2 < 4e-4
+  // (the series of -p0 RAD2DEG^2 4 asin^2(sqrt(c2)/2) * 64/ln2), else k_asin * asin^2(sqrt(c2)/2)
+  double k0, k1, k2, k3, k_asin;
+  uint32_t side;
+  uint32_t factor;
+};
+
+struct GridBuffers {
+  const hcg_sample* rec;
+  const uint32_t* keys;
+  const double* lon_s;
+  const double* lat_s;
+  const double* w_s;
+  const uint32_t* lut;
+  const double* ring_z;
+  const double* ring_s;
+  const hcg_ring_meta* meta;
+  const double* xwcs;
+  const double* ywcs;
+  double* dc;
+  double* wc;
+};
+
+// The window of an anchor cell (HCGrid/gridding.cu:306-323).
+struct Window {
+  uint32_t uring, dring;
+  double phi;  // anchor longitude, radians
+  bool polar;
+};
+
+__device__ inline Window window_of(const GridParams& P, uint32_t firstring, uint32_t usedrings, double xa_deg,
+                                   double ya_deg) {
+  Window w;
+  const double l1 = xa_deg * kDeg2Rad;
+  const double b1 = ya_deg * kDeg2Rad;
+  const double disc_theta = kHalfPi - b1;
+  double utheta = disc_theta - P.disc_size;
+  if (utheta * kRad2Deg < 0) utheta = 0;
+  uint64_t uring = pix2ring(P.hp, ang2pix(P.hp, utheta, l1));
+  if (uring < firstring) uring = firstring;
+  utheta = disc_theta + P.disc_size;
+  uint64_t dring = pix2ring(P.hp, ang2pix(P.hp, utheta, l1));
+  if (dring >= (uint64_t)firstring + usedrings) dring = (uint64_t)firstring + usedrings - 1;
+  w.uring = (uint32_t)uring;
+  w.dring = (uint32_t)dring;
+  w.phi = l1;
+  w.polar = (disc_theta * kRad2Deg <= kNorthB) || (disc_theta * kRad2Deg >= kSouthB);
+  return w;
+}
+
+// lpix/rpix of ring r (HCGrid/gridding.cu:336-360).  Returns false where the
+// reference would `continue` without advancing (an endless loop there); the
+// ring is skipped here instead.  ang2pix(theta_r, phi) == loc2pix(cos theta_r, ...).
+__device__ inline bool ring_window(const GridParams& P, const Window& w, uint64_t ring, double rz, double rs,
+                                   uint64_t& lpix, uint64_t& rpix) {
+  uint64_t sp, npr;
+  ring_info(P.hp, ring, sp, npr);
+  if (w.polar) {
+    lpix = sp;
+    rpix = sp + npr - 1;
+    return true;
+  }
+  const bool have = rs >= 0.0;
+  lpix = loc2pix(P.hp, rz, w.phi - P.disc_size, have ? rs : 0., have);
+  if (!(lpix >= sp && lpix < sp + npr)) return false;
+  rpix = loc2pix(P.hp, rz, w.phi + P.disc_size, have ? rs : 0., have);
+  if (!(rpix >= sp && rpix < sp + npr)) return false;
+  return true;
+}
+
+// ---- out-of-line slow paths (rare; kept out of the inner loop's registers) ----
+
+// The reference's radius test (HCGrid/helpers.cu:128-130, gridding.cu:381-386)
+// for a pair inside the guard band.
+static __device__ __noinline__ bool radius_test_ref(const double* xwcs, const double* ywcs, const double* lon_s,
+                                             const double* lat_s, uint32_t cell, uint32_t i, double radius) {
+  const double l1 = xwcs[cell] * kDeg2Rad, b1 = ywcs[cell] * kDeg2Rad;
+  const double l2 = lon_s[i] * kDeg2Rad, b2 = lat_s[i] * kDeg2Rad;
+  const double a = sin((b1 - b2) / 2.);
+  const double c = sin((l1 - l2) / 2.);
+  const double d = 2 * asin(sqrt(a * a + cos(b1) * cos(b2) * (c * c))) * kRad2Deg;
+  return d < radius;
+}
+
+static __device__ __noinline__ double asin2_direct(double h) {
+  const double a = asin(sqrt(h));
+  return a * a;
+}
+
+__device__ inline double asin2_of_h(double h) {  // asin(sqrt(h))^2
+  if (h < 1e-4) {
+    const double t = fma(h, 4.0 / 35.0, 8.0 / 45.0);
+    const double u = fma(h, t, 1.0 / 3.0);
+    return h * fma(h, u, 1.0);
+  }
+  return asin2_direct(h);
+}
+
+// GAUSS1D exponent -p0 d^2 (d in degrees) scaled by 64/ln2, from c2 = 4 sin^2(d/2):
+// d^2 = (2 RAD2DEG)^2 asin^2(sqrt(c2)/2) and asin^2(sqrt h) = h (1 + h/3 + 8h^2/45 + 4h^3/35 + ...),
+// i.e. (c2/4)(1 + c2/12 + c2^2/90 + c2^3/560), truncation < 1e-17 relative for c2 < 4e-4.
+__device__ inline double gauss_exponent_scaled(const GridParams& P, double c2) {
+  if (c2 < 4e-4) return c2 * fma(c2, fma(c2, fma(c2, P.k3, P.k2), P.k1), P.k0);
+  return P.k_asin * asin2_direct(0.25 * c2);
+}
+
+// GAUSS2D / TAPERED_SINC weights (HCGrid/gridding.cu:242-256).
+template <int KT>
+static __device__ __noinline__ double weight_general(double p0, double p1, double p2, const double* xwcs,
+                                              const double* ywcs, const double* lon_s, const double* lat_s,
+                                              double h, uint32_t cell, uint32_t i) {
+  const double sdist = 2.0 * sqrt(asin2_of_h(h)) * kRad2Deg;
+  const double p[3] = {p0, p1, p2};
+  if constexpr (KT == HCG_GAUSS2D) {
+    const double l1 = xwcs[cell] * kDeg2Rad, b1 = ywcs[cell] * kDeg2Rad;
+    const double l2 = lon_s[i] * kDeg2Rad, b2 = lat_s[i] * kDeg2Rad;
+    const double dl = l2 - l1, cb2 = cos(b2);
+    const double bear = atan2(cb2 * sin(dl), cos(b1) * sin(b2) - sin(b1) * cb2 * cos(dl));
+    const double sb = sin(bear - p[2]), cbr = cos(bear - p[2]);
+    const double ell = p[0] * p[0] * (sb * sb) + p[1] * p[1] * (cbr * cbr);
+    const double q = sdist / p[0] / p[1];
+    return exp(-(q * q / 2. * ell));
+  } else {
+    const double arg = kPi * sdist / p[0];
+    const double t = arg / p[1];
+    const double x = arg / p[2];
+    const double sinc = (fabs(x) < 1.e-10) ? 1. : sin(x) / x;  // HCGrid/gridding.cu:230-235
+    return sinc * exp(t * t);  // exp(pow(-(arg/p1), 2)) as the reference writes it
+  }
+}
+
+// ---- wave-level helpers ----
+
+// 64-way search over keys[lo, hi): first index whose key >= K (UPPER=false) or
+// > K (UPPER=true).  Wave-uniform inputs and result.
+template <bool UPPER>
+__device__ inline uint32_t wave_search(const uint32_t* __restrict__ keys, uint32_t lo, uint32_t hi, uint32_t K) {
+  const uint32_t lane = __lane_id();
+  while (hi - lo > 64) {
+    const uint32_t step = (hi - lo + 63) / 64;
+    const uint32_t p = lo + lane * step;
+    const bool inr = p < hi;
+    const uint32_t kv = inr ? keys[p] : 0xFFFFFFFFu;
+    const bool ge = inr ? (UPPER ? kv > K : kv >= K) : true;
+    const uint64_t m = __ballot(ge);
+    if (m == 0) {  // all 64 probes in range and below K: the answer is past the last probe
+      lo = lo + 63 * step + 1;
+      continue;
+    }
+    const uint32_t f = (uint32_t)__ffsll((unsigned long long)m) - 1;
+    const uint32_t pf = lo + f * step;
+    if (f == 0) return lo;
+    const uint32_t nlo = lo + (f - 1) * step + 1;
+    hi = pf < hi ? pf : hi;
+    lo = nlo;
+  }
+  const uint32_t p = lo + lane;
+  const bool inr = p < hi;
+  const uint32_t kv = inr ? keys[p] : 0u;
+  const bool ge = inr && (UPPER ? kv > K : kv >= K);
+  const uint64_t m = __ballot(ge);
+  return m ? lo + (uint32_t)__ffsll((unsigned long long)m) - 1 : hi;
+}
+
+__device__ inline uint32_t wave_min_u32(uint32_t v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const uint32_t o = __shfl_xor(v, off, 64);
+    v = o < v ? o : v;
+  }
+  return v;
+}
+
+__device__ inline uint32_t wave_max_u32(uint32_t v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const uint32_t o = __shfl_xor(v, off, 64);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+
+// Kernel-argument form of the set-up (host side).
+inline GridParams make_grid_params(const hcg_healpix* hp, const hcg_kernel* kern, uint32_t side) {
+  GridParams P;
+  P.hp = *hp;
+  P.disc_size = kern->disc_size;
+  P.sphere_radius = kern->sphere_radius;
+  for (int i = 0; i < 3; ++i) P.params[i] = kern->params[i];
+  P.side = side;
+  P.factor = kern->factor;
+  // radius threshold on c2 = 4 sin^2(R/2), with a 1e-9 relative guard band
+  // where the reference's haversine decides (the unit-vector c2 is good to ~1e-12)
+  const double half = kern->sphere_radius * kDeg2Rad / 2.0;
+  const double cR = 4.0 * sin(half) * sin(half);
+  P.c_lo = cR * (1.0 - 1e-9);
+  P.c_hi = cR * (1.0 + 1e-9);
+  // -p0 d^2 = -p0 (2 RAD2DEG)^2 asin^2(sqrt(c2)/2); in units of ln2/64
+  const double inv_ln2_64 = 0x1.71547652b82fep+6;
+  P.k_asin = -4.0 * kRad2Deg * kRad2Deg * kern->params[0] * inv_ln2_64;
+  P.k0 = P.k_asin / 4.0;
+  P.k1 = P.k0 / 12.0;
+  P.k2 = P.k0 / 90.0;
+  P.k3 = P.k0 / 560.0;
+  return P;
+}
+
+}  // namespace hcg

@@ -49,6 +49,19 @@ __host__ __device__ inline double fmodulo(double v1, double v2) {  // HCGrid/hel
   return (t == v2) ? 0. : t;
 }
 
+// fmod(v, 4.0) without the library's iterative reduction.  v * 0.25 and
+// 4.0 * trunc(.) are exact, and |4 trunc(v/4)| lies in [|v|/2, |v|] whenever
+// it is non-zero, so the subtraction is exact (Sterbenz): the result equals
+// fmod(v, 4.0) bit for bit for every finite v (and is NaN for NaN/inf).
+__host__ __device__ inline double fmod4(double v) { return v - 4.0 * trunc(v * 0.25); }
+
+// fmodulo(v1, 4.0) as loc2pix uses it.
+__host__ __device__ inline double fmodulo4(double v1) {
+  if (v1 >= 0) return (v1 < 4.0) ? v1 : fmod4(v1);
+  double t = fmod4(v1) + 4.0;
+  return (t == 4.0) ? 0. : t;
+}
+
 // Ring start pixel and size (HCGrid/healpix.cu:68-105).
 __host__ __device__ inline void ring_info(const hcg_healpix& hp, uint64_t ring, uint64_t& startpix,
                                           uint64_t& npr) {
@@ -68,7 +81,8 @@ __host__ __device__ inline void ring_info(const hcg_healpix& hp, uint64_t ring, 
 // Ring of a pixel (HCGrid/healpix.cu:108-123).
 __host__ __device__ inline uint64_t pix2ring(const hcg_healpix& hp, uint64_t pix) {
   if (pix < hp.ncap) return (1 + isqrt_u64(1 + 2 * pix)) >> 1;
-  if (pix < hp.npix - hp.ncap) return (pix - hp.ncap) / (4 * hp.nside) + hp.nside;
+  // (pix - ncap) / (4 nside): nside = 2^order, so the division is a shift
+  if (pix < hp.npix - hp.ncap) return ((pix - hp.ncap) >> (hp.order + 2)) + hp.nside;
   return 4 * hp.nside - ((1 + isqrt_u64(2 * (hp.npix - pix) - 1)) >> 1);
 }
 
@@ -76,7 +90,7 @@ __host__ __device__ inline uint64_t pix2ring(const hcg_healpix& hp, uint64_t pix
 __host__ __device__ inline uint64_t loc2pix(const hcg_healpix& hp, double z, double phi,
                                             double sin_theta, bool have_sin) {
   const double za = fabs(z);
-  const double tt = fmodulo(phi * kInvHalfPi, 4.0);
+  const double tt = fmodulo4(phi * kInvHalfPi);
   const double dn = (double)hp.nside;
   if (za <= kTwoThird) {
     const uint64_t nl4 = 4 * hp.nside;
@@ -87,7 +101,7 @@ __host__ __device__ inline uint64_t loc2pix(const hcg_healpix& hp, double z, dou
     const uint64_t ir = hp.nside + 1 + jp - jm;
     const uint64_t kshift = 1 - (ir & 1);
     const uint64_t tsum = jp + jm + kshift + 1 + nl4 + nl4 - hp.nside;
-    const uint64_t ip = (hp.order > 0) ? ((tsum >> 1) & (nl4 - 1)) : ((tsum >> 1) % nl4);
+    const uint64_t ip = (tsum >> 1) & (nl4 - 1);  // nl4 is a power of two: % nl4 == & (nl4 - 1)
     return hp.ncap + (ir - 1) * nl4 + ip;
   }
   const double tp = tt - (double)((uint64_t)tt);

@@ -185,8 +185,9 @@ __global__ void __launch_bounds__(256) k_gather(uint32_t n, const uint32_t* __re
     uint32_t j = perm[i];
     if (j >= n) j = i;  // a permutation never holds this; keeps every read in bounds regardless
     const double lo = lon[j], la = lat[j];
-    const double v = (dtype == HCG_DTYPE_F64) ? static_cast<const double*>(data)[j]
-                                              : (double)static_cast<const float*>(data)[j];
+    const double v = !data                      ? 0.0  // multi-channel jobs carry values separately
+                     : (dtype == HCG_DTYPE_F64) ? static_cast<const double*>(data)[j]
+                                                : (double)static_cast<const float*>(data)[j];
     double sb, cb, sl, cl;
     sincos(la * kDeg2Rad, &sb, &cb);
     sincos(lo * kDeg2Rad, &sl, &cl);
@@ -338,7 +339,7 @@ extern "C" int hcg_gather_samples(uint32_t n, const uint32_t* perm, const double
                                   double* lon_sorted, double* lat_sorted, double* data_sorted,
                                   double* weights_sorted, void* stream) {
   if (n == 0) return HCG_OK;
-  HCG_REQUIRE(perm && lon_deg && lat_deg && data && records, "hcg_gather_samples: NULL argument");
+  HCG_REQUIRE(perm && lon_deg && lat_deg && records, "hcg_gather_samples: NULL argument");
   HCG_REQUIRE(data_dtype == HCG_DTYPE_F64 || data_dtype == HCG_DTYPE_F32,
               "hcg_gather_samples: data_dtype must be HCG_DTYPE_F64 or HCG_DTYPE_F32");
   const uint32_t blocks = std::min<uint32_t>(ceil_div_u32(n, 256), 8192);

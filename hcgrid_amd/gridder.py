@@ -199,3 +199,68 @@ def grid_plain(lon, lat, data, xwcs, ywcs, side: int, sigma: float, radius: floa
                datacube=g.buf.datacube.cpu().numpy().copy(), weightscube=g.buf.weightscube.cpu().numpy().copy(),
                map=g.buf.map.cpu().numpy().copy(), kern=kern, hp=hp)
     return out
+
+
+class CubeGridder:
+    """Multi-channel path: the HCGrid+ channel-by-channel job as one pass.
+
+    Keys, sort, LUT and records as the single-channel path; channel values
+    (bf16/f16, channel-major [nchan][n] or sample-major [n][nchan]) are
+    gathered into sorted channel-major rows and contracted on MFMA.
+    """
+
+    def __init__(self, n: int, side: int, nchan: int, kern: Kernel, hp: Healpix, dtype: str = "bf16",
+                 device: int = 0):
+        torch = _torch()
+        capi.load()
+        if not torch.cuda.is_available():
+            raise capi.HcgError(capi.HCG_ERR_HIP, "CubeGridder", "no GPU visible to torch")
+        if dtype not in ("bf16", "f16"):
+            raise capi.HcgError(capi.HCG_ERR_INVALID, "CubeGridder", "dtype must be bf16 or f16")
+        self.plain = PlainGridder(n, side, kern, hp, device=device)
+        self.n, self.side, self.nchan, self.kern, self.hp = int(n), int(side), int(nchan), kern, hp
+        self.dtype = dtype
+        self.dcode = capi.DTYPE_BF16 if dtype == "bf16" else capi.DTYPE_F16
+        tdt = torch.bfloat16 if dtype == "bf16" else torch.float16
+        dev = self.plain.device
+        self.n_pad = int(capi.load().hcg_channels_padded_len(self.n))
+        self.values_sorted = torch.empty((self.nchan, self.n_pad), dtype=tdt, device=dev)
+        self.cube = torch.empty((self.nchan, self.side * self.side), dtype=torch.float32, device=dev)
+        self.wsum = torch.empty(self.side * self.side, dtype=torch.float32, device=dev)
+        self.nonfinite = torch.zeros(1, dtype=torch.int32, device=dev)
+
+    def gather_channels(self, values, layout: int = 0, stream=None):
+        call("hcg_gather_channels", self.n, self.nchan, _ptr(self.plain.buf.perm), _ptr(values), self.dcode, layout,
+             _ptr(self.values_sorted), _ptr(self.nonfinite), _stream_handle(stream))
+
+    def grid(self, xwcs, ywcs, normalize: bool = True, stream=None):
+        b = self.plain.buf
+        call("hcg_grid_mfma", C.byref(self.hp), C.byref(self.kern), self.side, self.n, self.nchan, self.dcode,
+             _ptr(b.records), _ptr(b.keys), _ptr(b.lon_s), _ptr(b.lat_s), None, _ptr(self.values_sorted),
+             _ptr(b.lut), _ptr(b.ring_z), _ptr(b.ring_s), _ptr(b.meta), _ptr(xwcs), _ptr(ywcs), _ptr(self.cube),
+             _ptr(self.wsum), 1 if normalize else 0, _stream_handle(stream))
+
+    def run(self, lon, lat, values, xwcs, ywcs, layout: int = 0, stream=None, normalize: bool = True):
+        """One pass: keys + sort, records, LUT, channel gather, contraction."""
+        g = self.plain
+        g.presort(lon, lat, stream)
+        call("hcg_gather_samples", self.n, _ptr(g.buf.perm), _ptr(lon), _ptr(lat), None, capi.DTYPE_F64, None,
+             _ptr(g.buf.records), _ptr(g.buf.lon_s), _ptr(g.buf.lat_s), None, None, _stream_handle(stream))
+        g.ring_lut(stream)
+        self.gather_channels(values, layout, stream)
+        self.grid(xwcs, ywcs, normalize, stream)
+
+
+def grid_cube(lon, lat, values_cm, xwcs, ywcs, side: int, sigma: float, radius: float, hpx_res: float,
+              dtype: str = "bf16", layout: int = 0, device: int = 0) -> dict:
+    """Host arrays in/out: multi-channel pass (values_cm: torch tensor on the device)."""
+    torch = _torch()
+    kern, hp = prepare_grid_kernel(GAUSS1D, [sigma], radius, hpx_res, 1)
+    nchan = values_cm.shape[0] if layout == 0 else values_cm.shape[1]
+    g = CubeGridder(len(lon), side, nchan, kern, hp, dtype=dtype, device=device)
+    dev = g.plain.device
+    t = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device=dev)  # noqa: E731
+    g.run(t(lon), t(lat), values_cm.contiguous(), t(xwcs), t(ywcs), layout=layout)
+    torch.cuda.synchronize()
+    return dict(cube=g.cube.float().cpu().numpy().copy(), wsum=g.wsum.cpu().numpy().copy(),
+                perm=g.plain.perm(), nonfinite=int(g.nonfinite.item()), meta=g.plain.meta())

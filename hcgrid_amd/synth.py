@@ -109,3 +109,71 @@ def raster_small(n_lines: int, n_per_line: int, side: int, dec_c: float = 30.0, 
                               spacing_ra / 6.4, n_src, seed)
     return Workload(f"raster_{n_lines}x{n_per_line}", lon, lat, val,
                     Target(side, cdelt, ra_c, dec_c, side / 2.0))
+
+
+@dataclasses.dataclass
+class CubeWorkload:
+    name: str
+    lon: np.ndarray          # f64 deg
+    lat: np.ndarray          # f64 deg
+    values: object           # torch tensor (nchan, n) bf16 on the device, channel-major
+    target: Target
+    nchan: int
+    sigma: float = 0.02
+    radius: float = 0.06
+    hpx_res: float = 0.01
+
+    @property
+    def n(self) -> int:
+        return int(self.lon.size)
+
+
+def raster_cube(n_lines: int = 500, n_per_line: int = 4000, dec0: float = 25.01, ddec: float = 0.02,
+                ra0: float = 175.00125, dra: float = 0.0025, nchan: int = 1024, side: int = 1024,
+                n_src: int = 20, seed: int = 2, device: str = "cuda", dtype: str = "bf16") -> CubeWorkload:
+    """Spectral raster: per-source Gaussian line profiles (centre U[200, 800]
+    channels scaled to nchan/1024, width U[5, 30], amplitude U[1, 10]) on a
+    0.025 deg beam, plus N(0, 0.5) noise.  Default: 500 lines x 4000 samples
+    over 10 x 10 deg at (180, 30), 1024 channels, 1024 x 1024 map at 0.01 deg.
+    Values are generated on the device (torch, seeded) in channel-major order."""
+    import torch
+
+    rng = np.random.Generator(np.random.PCG64(seed))
+    k = np.arange(n_lines, dtype=np.float64)
+    j = np.arange(n_per_line, dtype=np.float64)
+    lat = np.repeat(dec0 + ddec * k, n_per_line)
+    lon = np.tile(ra0 + dra * j, n_lines) + rng.uniform(-dra / 6.4, dra / 6.4, size=n_lines * n_per_line)
+    n = lon.size
+    tdt = torch.bfloat16 if dtype == "bf16" else torch.float16
+    gen = torch.Generator(device=device)
+    gen.manual_seed(seed)
+    vals = torch.empty((nchan, n), dtype=tdt, device=device)
+    chunk = max(1, (1 << 26) // max(n, 1))
+    for c0 in range(0, nchan, chunk):
+        c1 = min(nchan, c0 + chunk)
+        vals[c0:c1] = (0.5 * torch.randn((c1 - c0, n), generator=gen, device=device)).to(tdt)
+    amp = rng.uniform(1.0, 10.0, n_src)
+    sra = rng.uniform(ra0, ra0 + dra * (n_per_line - 1), n_src)
+    sde = rng.uniform(dec0, dec0 + ddec * (n_lines - 1), n_src)
+    mu = rng.uniform(200.0, 800.0, n_src) * (nchan / 1024.0)
+    wid = rng.uniform(5.0, 30.0, n_src) * max(nchan / 1024.0, 1.0 / 30.0)
+    reach = 6.0 * BEAM_SIGMA_DEG
+    ch = torch.arange(nchan, device=device, dtype=torch.float32)[:, None]
+    for a, r, d, m_, w_ in zip(amp, sra, sde, mu, wid):
+        rows = np.nonzero(np.abs(dec0 + ddec * k - d) < reach)[0]
+        if rows.size == 0:
+            continue
+        sl = slice(int(rows[0]) * n_per_line, (int(rows[-1]) + 1) * n_per_line)
+        dist = _haversine_deg(lon[sl], lat[sl], r, d)
+        idx = np.nonzero(dist < reach)[0] + sl.start
+        if idx.size == 0:
+            continue
+        spat = a * np.exp(-dist[idx - sl.start] ** 2 / (2.0 * BEAM_SIGMA_DEG ** 2))
+        prof = torch.exp(-((ch - float(m_)) ** 2) / (2.0 * float(w_) ** 2))
+        add = prof * torch.as_tensor(spat, dtype=torch.float32, device=device)[None, :]
+        it = torch.as_tensor(idx, device=device)
+        vals[:, it] = (vals[:, it].float() + add).to(tdt)
+    dec_c = dec0 + ddec * (n_lines - 1) / 2.0
+    ra_c = ra0 + dra * (n_per_line - 1) / 2.0
+    return CubeWorkload(f"raster_cube_{n // 1000}k_x{nchan}", lon, lat, vals,
+                        Target(side, 0.01, round(ra_c, 6), round(dec_c, 6), side / 2.0), nchan)

@@ -172,11 +172,49 @@ int hcg_grid_plain(const hcg_healpix *hp, const hcg_kernel *kern, uint32_t side,
                    const double *ring_s, const hcg_ring_meta *meta, const double *xwcs,
                    const double *ywcs, double *datacube, double *weightscube, void *stream);
 
+/* ---- multi-channel (spectral cube) path ---- */
+
+/* Row length (elements) of the sorted channel buffer: n rounded up to 8 plus
+ * 64 zero elements, so the contraction can read whole 32-sample blocks. */
+uint32_t hcg_channels_padded_len(uint32_t n);
+
+/* The per-channel permutation gather of HCGrid+'s pre_order_data
+ * (HCGrid+/gridding-omp.cu:146-157), all channels at once: values (dtype
+ * HCG_DTYPE_BF16 or HCG_DTYPE_F16) laid out channel-major [nchan][n]
+ * (layout 0, the reference's per-channel arrays) or sample-major
+ * [n][nchan] (layout 1) -> values_sorted [nchan][hcg_channels_padded_len(n)]
+ * in sorted sample order, zero-padded.  nonfinite (device, may be NULL)
+ * is incremented once per non-finite value: the contraction multiplies
+ * every staged value, so non-finite channel values are not supported. */
+int hcg_gather_channels(uint32_t n, uint32_t nchan, const uint32_t *perm, const void *values, int dtype,
+                        int layout, void *values_sorted, uint32_t *nonfinite, void *stream);
+
+/* Multi-channel gridding: what HCGrid+ computes by running `hcgrid` once per
+ * channel (HCGrid+/gridding-omp.cu:552-678), as one pass.  Same selection
+ * and FP64 weights as hcg_grid_plain; the weights are rounded to dtype and
+ * contracted with the channel values on MFMA with FP32 accumulation:
+ *   cube[ch][c] = sum_s w~(c,s) v[ch][s] / sum_s w~(c,s)   (normalize != 0)
+ * or the raw numerator (normalize == 0), w~ = weight rounded to dtype.
+ * cube is FP32 [nchan][side*side] (channel-major, like the reference's
+ * datacube planes); wsum (FP32 [side*side], may be NULL) receives sum w~.
+ * NaN where a cell has no weight.  Samples are the sorted records/keys of
+ * hcg_presort/hcg_gather_samples; values_sorted from hcg_gather_channels. */
+int hcg_grid_mfma(const hcg_healpix *hp, const hcg_kernel *kern, uint32_t side, uint32_t n, uint32_t nchan,
+                  int dtype, const hcg_sample *records, const uint32_t *keys_sorted,
+                  const double *lon_sorted, const double *lat_sorted, const double *weights_sorted,
+                  const void *values_sorted, const uint32_t *lut, const double *ring_z,
+                  const double *ring_s, const hcg_ring_meta *meta, const double *xwcs,
+                  const double *ywcs, float *cube, float *wsum, int normalize, void *stream);
+
 /* Output normalisation of write_output_map (HCGrid/gmap.cpp:305-312):
  * map[c] = datacube[c] / weightscube[c] in FP64, NaN where the weight is 0.
  * map may alias datacube. */
 int hcg_normalize(uint64_t ncells, const double *datacube, const double *weightscube,
                   double *map, void *stream);
+
+/* The device exponential the gridding kernels use for GAUSS1D weights,
+ * evaluated elementwise (x <= 0): a validation hook, no reference counterpart. */
+int hcg_eval_exp_neg(uint32_t n, const double *x, double *out, void *stream);
 
 /* ---- host-side target geometry ---- */
 /* set_WCS for a SIN (orthographic) target (HCGrid/gmap.cpp:183-250): cell

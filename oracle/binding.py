@@ -94,6 +94,10 @@ def lib():
         L.orc_grid_compat_rows.argtypes = [P(OrcHpx), P(OrcGrid), C.c_uint32, vp, vp, vp, vp, vp,
                                            C.c_uint64, C.c_uint32, vp, vp, vp, vp, vp,
                                            C.c_uint32, C.c_uint32, C.c_int]
+        L.orc_grid_compat_multi.restype = C.c_int64
+        L.orc_grid_compat_multi.argtypes = [P(OrcHpx), P(OrcGrid), C.c_uint32, vp, vp, C.c_uint32, vp, vp,
+                                            vp, C.c_uint64, C.c_uint32, vp, vp, vp, C.c_uint32, C.c_uint32,
+                                            vp, vp, C.c_int]
         L.orc_grid_bruteforce.restype = None
         L.orc_grid_bruteforce.argtypes = [P(OrcHpx), P(OrcGrid), C.c_uint32, vp, vp, vp, vp, vp,
                                           C.c_uint64, C.c_uint32, vp, vp, vp, vp, C.c_int]
@@ -250,6 +254,31 @@ def grid_compat_rows(h, g, lon_s, lat_s, data_s, weights_s, keys_sorted, firstri
                                _p(xw), _p(yw), _p(dc), _p(wc), row_begin, row_end, nthreads)
     sl = slice(row_begin * side, row_end * side)
     return dc[sl].copy(), wc[sl].copy()
+
+
+def grid_compat_multi(h, g, lon_s, lat_s, values_cm_s, weights_s, keys_sorted, firstring, usedrings, lut,
+                      xwcs, ywcs, row_begin, row_end, nthreads=0):
+    """Multi-channel restated kernel over rows [row_begin, row_end).
+
+    values_cm_s: (nchan, n) float32 channel-major, in sorted order.
+    Returns (sums (nchan, rows*side), wsum (rows*side,))."""
+    v = np.ascontiguousarray(values_cm_s, dtype=np.float32)
+    nchan, n = v.shape
+    side = int(g.side)
+    ncr = (row_end - row_begin) * side
+    lon_s = np.ascontiguousarray(lon_s, dtype=np.float64)
+    lat_s = np.ascontiguousarray(lat_s, dtype=np.float64)
+    w = None if weights_s is None else np.ascontiguousarray(weights_s, dtype=np.float64)
+    keys = np.ascontiguousarray(keys_sorted, dtype=np.uint64)
+    lut = np.ascontiguousarray(lut, dtype=np.uint32)
+    xw = np.ascontiguousarray(xwcs, dtype=np.float64)
+    yw = np.ascontiguousarray(ywcs, dtype=np.float64)
+    out = np.zeros((nchan, ncr))
+    ws = np.zeros(ncr)
+    lib().orc_grid_compat_multi(C.byref(h), C.byref(g), n, _p(lon_s), _p(lat_s), nchan, _p(v),
+                                None if w is None else _p(w), _p(keys), firstring, usedrings, _p(lut),
+                                _p(xw), _p(yw), row_begin, row_end, _p(out), _p(ws), nthreads)
+    return out, ws
 
 
 def grid_bruteforce(h, g, lon_s, lat_s, data_s, weights_s, keys_sorted, firstring, usedrings,

@@ -504,6 +504,97 @@ int64_t orc_grid_compat_rows(const orc_hpx *h, const orc_grid *g, uint32_t n,
     return hangs;
 }
 
+/* Multi-channel form: the selection and FP64 weights of orc_grid_compat for
+ * every cell of rows [row_begin, row_end), applied to nchan channels of
+ * channel-major sorted values (values[ch * n + i], the layout of HCGrid+'s
+ * per-channel arrays, HCGrid+/gridding-omp.cu:146-157).  Writes raw sums:
+ * out[ch * ncells_rows + k] (k = cell - row_begin * side) and wsum[k].  This
+ * is what the reference's channel-by-channel pipeline computes, one channel
+ * at a time (HCGrid+/gridding-omp.cu:616-626). */
+int64_t orc_grid_compat_multi(const orc_hpx *h, const orc_grid *g, uint32_t n,
+                              const double *lon, const double *lat, uint32_t nchan, const float *values,
+                              const double *weights, const uint64_t *keys,
+                              uint64_t firstring, uint32_t usedrings, const uint32_t *lut,
+                              const double *xwcs, const double *ywcs,
+                              uint32_t row_begin, uint32_t row_end, double *out, double *wsum,
+                              int nthreads) {
+    const uint32_t side = g->side;
+    if (row_end > side) row_end = side;
+    const uint64_t ncr = (uint64_t)(row_end - row_begin) * side;
+    int64_t hangs = 0;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+    if (n == 0 || usedrings == 0) return 0;
+#pragma omp parallel reduction(+ : hangs)
+    {
+        uint32_t cap = 4096;
+        uint32_t *sel = (uint32_t *)malloc(sizeof(uint32_t) * cap);
+        double *wts = (double *)malloc(sizeof(double) * cap);
+        double *acc = (double *)malloc(sizeof(double) * (nchan ? nchan : 1));
+#pragma omp for schedule(dynamic, 4)
+        for (int64_t k = 0; k < (int64_t)ncr; ++k) {
+            uint64_t c = (uint64_t)row_begin * side + (uint64_t)k;
+            uint32_t row = (uint32_t)(c / side), col = (uint32_t)(c % side);
+            uint32_t factor = g->factor ? g->factor : 1;
+            uint64_t anchor = (uint64_t)row * side + (col / factor) * factor;
+            orc_window w;
+            orc_window_of(h, g, firstring, usedrings, xwcs[anchor], ywcs[anchor], &w);
+            double l1 = xwcs[c] * ORC_DEG2RAD, b1 = ywcs[c] * ORC_DEG2RAD;
+            uint32_t m = 0;
+            double sw = 0.0;
+            uint64_t uring = w.uring;
+            uint32_t start_int = lut[uring - firstring];
+            while (uring <= w.dring) {
+                uint32_t end_int = lut[uring - firstring + 1];
+                uint64_t lpix, rpix;
+                if (!orc_ring_window(h, g, &w, uring, &lpix, &rpix)) {
+                    hangs++;
+                    start_int = end_int;
+                    ++uring;
+                    continue;
+                }
+                uint32_t i = orc_search_last_less(keys, start_int - 1u, end_int, lpix);
+                ++i;
+                if (i > end_int) i = n;
+                for (; i < n; ++i) {
+                    double l2 = lon[i] * ORC_DEG2RAD, b2 = lat[i] * ORC_DEG2RAD;
+                    if (orc_ang2pix(h, ORC_HALFPI - b2, l2) > rpix) break;
+                    double sdist = orc_true_angular_distance(l1, b1, l2, b2) * ORC_RAD2DEG;
+                    if (!(sdist < g->sphere_radius)) continue;
+                    double sbear = g->bearing_needed ? orc_great_circle_bearing(l1, b1, l2, b2) : 0.;
+                    double tw = (weights ? weights[i] : 1.0) * orc_kernel_weight(g, sdist, sbear);
+                    if (m == cap) {
+                        cap *= 2;
+                        sel = (uint32_t *)realloc(sel, sizeof(uint32_t) * cap);
+                        wts = (double *)realloc(wts, sizeof(double) * cap);
+                    }
+                    sel[m] = i;
+                    wts[m] = tw;
+                    ++m;
+                    sw += tw;
+                }
+                start_int = end_int;
+                ++uring;
+            }
+            for (uint32_t ch = 0; ch < nchan; ++ch) {
+                const float *vch = values + (uint64_t)ch * n;
+                double a = 0.0;
+                for (uint32_t q = 0; q < m; ++q) a += (double)vch[sel[q]] * wts[q];
+                acc[ch] = a;
+            }
+            for (uint32_t ch = 0; ch < nchan; ++ch) out[(uint64_t)ch * ncr + (uint64_t)k] = acc[ch];
+            wsum[k] = sw;
+        }
+        free(sel);
+        free(wts);
+        free(acc);
+    }
+    return hangs;
+}
+
 void orc_grid_bruteforce(const orc_hpx *h, const orc_grid *g, uint32_t n,
                          const double *lon, const double *lat, const double *data,
                          const double *weights, const uint64_t *keys,

@@ -112,6 +112,18 @@ int64_t orc_grid_compat_rows(const orc_hpx *h, const orc_grid *g, uint32_t n,
                              double *datacube, double *weightscube,
                              uint32_t row_begin, uint32_t row_end, int nthreads);
 
+/* Multi-channel form over rows [row_begin, row_end): FP64 weights of the
+ * same selection applied to nchan channel-major float values
+ * (values[ch * n + i], sorted order).  out[ch * rows*side + k] = sum w v,
+ * wsum[k] = sum w (raw sums, no normalisation). */
+int64_t orc_grid_compat_multi(const orc_hpx *h, const orc_grid *g, uint32_t n,
+                              const double *lon, const double *lat, uint32_t nchan, const float *values,
+                              const double *weights, const uint64_t *keys_sorted,
+                              uint64_t firstring, uint32_t usedrings, const uint32_t *lut,
+                              const double *xwcs, const double *ywcs,
+                              uint32_t row_begin, uint32_t row_end, double *out, double *wsum,
+                              int nthreads);
+
 /* Same selection, evaluated pair by pair over all samples (no LUT, no search). */
 void orc_grid_bruteforce(const orc_hpx *h, const orc_grid *g, uint32_t n,
                          const double *lon, const double *lat, const double *data,
