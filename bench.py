@@ -11,6 +11,10 @@ Prints one JSON line (rank 0).  `value` = samples x channels / step time.
 The roofline object covers the gridding kernel (the dominant launch), timed
 with HIP events on the stream it is launched on; the CPU baseline is the
 oracle (the C/OpenMP restatement) on a bounded sample of the same workload.
+
+The `multichannel` object (on by default, --no-cube to skip) measures the
+MFMA path on a 2M-sample x 1024-channel BF16 raster onto a 1024 x 1024 map
+and the single-channel path applied channel by channel to the same samples.
 """
 from __future__ import annotations
 
@@ -36,7 +40,9 @@ def parse():
     ap.add_argument("--workload", default="raster_20m", choices=["raster_20m", "raster_250k"])
     ap.add_argument("--cpu-baseline", dest="cpu_baseline", action="store_true", default=True)
     ap.add_argument("--no-cpu-baseline", dest="cpu_baseline", action="store_false")
-    ap.add_argument("--check", action="store_true", help="compare a band of rows with the oracle")
+    ap.add_argument("--cube", dest="cube", action="store_true", default=True,
+                    help="also measure the multi-channel path (2M samples x 1024 BF16 channels)")
+    ap.add_argument("--no-cube", dest="cube", action="store_false")
     return ap.parse_args()
 
 
@@ -82,6 +88,144 @@ def cpu_baseline(wl, xw, yw, log):
         "sample": (f"{wl.name}: keys, stable sort, gathers and ring LUT over all {wl.n} samples; "
                    f"gridding of rows {r0}-{r0 + rows - 1} ({rows} of {side}), time scaled x{side / rows:g} "
                    f"to the full map"),
+    }
+
+
+def cube_algorithmic_bytes(n: int, m: int, nchan: int) -> int:
+    # channel values (bf16) read once; per sample a 32 B record + 4 B key;
+    # per cell the target coordinates (16 B) and the weight sum (4 B); the
+    # FP32 cube written once
+    return n * nchan * 2 + n * 36 + m * 20 + m * nchan * 4
+
+
+def bench_cube(args, log):
+    """Multi-channel path on a 2M-sample x 1024-channel BF16 raster onto a
+    1024 x 1024 map, against the single-channel path applied channel by
+    channel to the same samples (what the reference's channel loop does: one
+    pre-sort, then per channel a value gather, zeroed cubes, the gridding
+    kernel and the normalisation)."""
+    import torch
+
+    from hcgrid_amd import gridder, synth
+
+    dev = torch.device("cuda", 0)
+    stream = torch.cuda.current_stream()
+    t0 = time.perf_counter()
+    wl = synth.raster_cube(device="cuda")
+    t = wl.target
+    xw, yw = gridder.wcs_sin(t.side, t.side, t.crpix, t.crpix, -t.cdelt, t.cdelt, t.crval1, t.crval2)
+    torch.cuda.synchronize()
+    log(f"cube workload {wl.name}: n={wl.n} nchan={wl.nchan} side={t.side} generated in "
+        f"{time.perf_counter() - t0:.1f}s")
+    kern, hp = gridder.prepare_grid_kernel(1, [wl.sigma], wl.radius, wl.hpx_res)
+    cg = gridder.CubeGridder(wl.n, t.side, wl.nchan, kern, hp, dtype="bf16")
+    lon = torch.as_tensor(wl.lon, device=dev)
+    lat = torch.as_tensor(wl.lat, device=dev)
+    xwd = torch.as_tensor(xw, device=dev)
+    ywd = torch.as_tensor(yw, device=dev)
+    vals = wl.values
+
+    def cube_step(ev=None):
+        if ev is not None:
+            ev[0].record(stream)
+        cg.plain.presort(lon, lat)
+        cg.gather_records(lon, lat)
+        cg.plain.ring_lut()
+        cg.gather_channels(vals)
+        if ev is not None:
+            ev[1].record(stream)
+        cg.grid(xwd, ywd)
+        if ev is not None:
+            ev[2].record(stream)
+
+    for _ in range(args.warmup):
+        cube_step()
+    torch.cuda.synchronize()
+    evs = [tuple(torch.cuda.Event(enable_timing=True) for _ in range(3)) for _ in range(args.steps)]
+    for i in range(args.steps):
+        cube_step(evs[i])
+    torch.cuda.synchronize()
+    step_ms = [a.elapsed_time(c) for a, b, c in evs]
+    grid_ms = [b.elapsed_time(c) for a, b, c in evs]
+    ms_step = sum(step_ms) / len(step_ms)
+    ms_grid = sum(grid_ms) / len(grid_ms)
+    updates = wl.n * wl.nchan
+    log(f"cube: {ms_step:.3f} ms/step, k_grid_mfma avg {ms_grid:.3f} ms")
+
+    # the single-channel path, channel by channel, on the same samples
+    pg = gridder.PlainGridder(wl.n, t.side, kern, hp)
+    ch0 = vals[0].float().contiguous()
+    pg.presort(lon, lat)
+    pg.ring_lut()
+
+    def channel_step(ev=None):
+        if ev is not None:
+            ev[0].record(stream)
+        pg.gather(lon, lat, ch0)
+        pg.init_output()
+        if ev is not None:
+            ev[1].record(stream)
+        pg.grid(xwd, ywd)
+        if ev is not None:
+            ev[2].record(stream)
+        pg.normalize()
+        if ev is not None:
+            ev[3].record(stream)
+
+    for _ in range(args.warmup):
+        channel_step()
+    torch.cuda.synchronize()
+    evp = [tuple(torch.cuda.Event(enable_timing=True) for _ in range(4)) for _ in range(args.steps)]
+    for i in range(args.steps):
+        channel_step(evp[i])
+    torch.cuda.synchronize()
+    ch_ms = sum(a.elapsed_time(d) for a, b, c, d in evp) / args.steps
+    ch_grid = sum(b.elapsed_time(c) for a, b, c, d in evp) / args.steps
+    # one pre-sort + LUT, then the per-channel work for every channel
+    ps = [tuple(torch.cuda.Event(enable_timing=True) for _ in range(2)) for _ in range(args.steps)]
+    for i in range(args.steps):
+        ps[i][0].record(stream)
+        pg.presort(lon, lat)
+        pg.ring_lut()
+        ps[i][1].record(stream)
+    torch.cuda.synchronize()
+    presort_ms = sum(a.elapsed_time(b) for a, b in ps) / args.steps
+    chan_total_ms = presort_ms + wl.nchan * ch_ms
+    chan_rate = updates / (chan_total_ms * 1e-3)
+    cube_rate = updates / (ms_step * 1e-3)
+    log(f"channel by channel: pre-sort {presort_ms:.3f} ms + {wl.nchan} x {ch_ms:.3f} ms "
+        f"(k_grid_tiles {ch_grid:.3f} ms) = {chan_total_ms:.1f} ms; ratio {cube_rate / chan_rate:.2f}x")
+
+    m = t.side * t.side
+    abytes = cube_algorithmic_bytes(wl.n, m, wl.nchan)
+    achieved = abytes / (ms_grid * 1e-3) / 1e9
+    return {
+        "workload": (f"{wl.name}: {wl.n} samples (raster scan, 500 lines x 4000) x {wl.nchan} BF16 channels, "
+                     f"{t.side}x{t.side} SIN map at 0.01 deg, GAUSS1D sigma {wl.sigma} deg, radius {wl.radius} deg, "
+                     f"nside {hp.nside}; FP64 weights rounded to BF16, FP32 accumulation"),
+        "value": cube_rate,
+        "unit": "sample·channel updates/s",
+        "ms_per_step": ms_step,
+        "dtype": "bf16",
+        "channel_by_channel": {
+            "value": chan_rate,
+            "ms_total": chan_total_ms,
+            "presort_ms": presort_ms,
+            "per_channel_ms": ch_ms,
+            "per_channel_grid_ms": ch_grid,
+        },
+        "speedup_vs_channel_by_channel": cube_rate / chan_rate,
+        "roofline": {
+            "kernel": "k_grid_mfma",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "algorithmic_bytes": abytes,
+            "avg_ms": ms_grid,
+        },
     }
 
 
@@ -180,7 +324,7 @@ def main():
             "usedrings": meta["usedrings"],
         },
         "roofline": {
-            "kernel": "k_grid_cells",
+            "kernel": "k_grid_tiles",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
@@ -193,18 +337,12 @@ def main():
         },
     }
 
-    if args.check:
-        import oracle
-        from tests.helpers import map_errors  # noqa: F401
-        log("check: oracle on a band of rows")
-        h2, g2 = oracle.prepare_grid_kernel(oracle.GAUSS1D, [wl.sigma], wl.radius, wl.hpx_res, t.side)
-        keys, perm = oracle.presort(h2, wl.lon, wl.lat)
-        assert np.array_equal(g.keys_sorted().astype(np.uint64), keys)
-        assert np.array_equal(g.perm(), perm)
-        log("check: keys and order bit-exact")
-
     if args.cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(wl, xw, yw, log)
+    if args.cube:
+        del g, lon, lat, val, xwd, ywd
+        torch.cuda.empty_cache()
+        result["multichannel"] = bench_cube(args, log)
     print(json.dumps(result), flush=True)
 
 

@@ -6,19 +6,25 @@
 // The selection and weights do not depend on the channel, so here they are
 // computed once per (cell, sample) pair in FP64 and the channel sums become
 //     cube[ch][cell] = sum_s W[cell][s] * V[ch][s]   (FP32 accumulation)
-// evaluated with v_mfma_f32_16x16x32_{bf16,f16}: W (cells x samples, rounded
-// to the value type) in LDS, V (channels x sorted samples) streamed from HBM.
+// evaluated with v_mfma_f32_16x16x32_{bf16,f16}: W (cells x samples) in LDS,
+// V (channels x sorted samples) streamed from HBM.  Each FP64 weight is
+// carried as hi + lo, two values of the channel type (hi = round(w),
+// lo = round(w - hi)), and both are contracted: the weight keeps ~2x the
+// type's significand (relative error ~2^-17 for BF16) at twice the MFMA
+// count, which the kernel's HBM/L2 stream of V hides.
 //
 // Work decomposition: a workgroup owns one map row x 64 columns (4 MFMA
-// row blocks of 16 cells) and up to 8 x 128 channels (wave w: 8 blocks of
-// 16 channels, 128 accumulators per lane).  For every non-empty ring of the
+// row blocks of 16 cells) and up to 8 x 64 channels (wave w: 4 blocks of
+// 16 channels; 4 row blocks x 4 channel blocks x 4 = 64 FP32 accumulators
+// per lane).  Wider channel ranges take more workgroups along grid.y.
+// For every non-empty ring of the
 // row segment's ring range:
 //   1. each cell's [lpix, rpix] window (the reference's compat selection);
 //   2. the union window's sorted range by a 64-way cooperative key search;
-//   3. per chunk of <= 512 samples: stage unit vectors and keys in LDS, zero
-//      the weight tile, let every cell's threads fill its band with the
-//      FP64-decided weights (rounded to the value type; their rounded sum
-//      is the normalisation, so numerator and denominator use the same
+//   3. per chunk of <= CHUNK samples: stage unit vectors and keys in LDS,
+//      zero the weight tiles, let every cell's threads fill its band with
+//      the FP64-decided weights (as hi + lo; the sum of the rounded pairs is
+//      the normalisation, so numerator and denominator use the same
 //      weights), record each row block's k-block span;
 //   4. MFMA over the union's 32-sample k-blocks: B (16 channels x 32
 //      samples) is loaded once per k-block and applied to every row block
@@ -36,8 +42,6 @@ typedef _Float16 f16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
 
 constexpr int kMcCols = 64;                   // cells per workgroup (one map row)
-constexpr uint32_t kMcChunk = 512;            // staged samples per chunk
-constexpr uint32_t kMcWStride = kMcChunk + 8;  // weight-tile row stride (elements), 16-byte padding
 constexpr int kMcNB = 4;                      // 16-channel blocks per wave
 constexpr int kMcMaxWaves = 8;
 constexpr uint32_t kMcChanPerWave = 16 * kMcNB;
@@ -70,15 +74,20 @@ __device__ inline double pair_weight_any(const GridParams& P, const GridBuffers&
                                           B.lat_s, 0.25 * c2, cell, i);
 }
 
-template <typename T>
+// CHUNK = staged samples per chunk.  LDS: two weight tiles of 64 x (CHUNK+8)
+// elements (16-byte row padding) + CHUNK unit vectors and keys: 148 KB at
+// 512 (one 8-wave workgroup per CU), 75 KB at 256, 38 KB at 128.
+template <typename T, uint32_t CHUNK>
 __global__ void __launch_bounds__(64 * kMcMaxWaves)
     k_grid_mfma(GridParams P, GridBuffers B, uint32_t kt, const T* __restrict__ vs, uint32_t n_pad, uint32_t nchan,
                 uint32_t segs_x, uint32_t nsegs, float* __restrict__ cube, float* __restrict__ wsum_out,
                 int normalize) {
   typedef typename Mfma<T>::vec vec;
-  __shared__ __attribute__((aligned(16))) T s_w[kMcCols * kMcWStride];
-  __shared__ double s_x[kMcChunk], s_y[kMcChunk], s_z[kMcChunk];
-  __shared__ uint32_t s_key[kMcChunk];
+  constexpr uint32_t WS = CHUNK + 8;  // weight-tile row stride (elements)
+  __shared__ __attribute__((aligned(16))) T s_wh[kMcCols * WS];
+  __shared__ __attribute__((aligned(16))) T s_wl[kMcCols * WS];
+  __shared__ double s_x[CHUNK], s_y[CHUNK], s_z[CHUNK];
+  __shared__ uint32_t s_key[CHUNK];
   __shared__ double s_exp[64];
   __shared__ uint32_t s_mb[4][2];
   __shared__ uint32_t s_red[kMcMaxWaves][2];
@@ -117,7 +126,7 @@ __global__ void __launch_bounds__(64 * kMcMaxWaves)
   const double px = cb0 * cl, py = cb0 * sl, pz = sb;
   double wsum = 0.0;
 
-  // accumulators: 4 row blocks x 8 channel blocks of 16x16
+  // accumulators: 4 row blocks x kMcNB channel blocks of 16x16
   f32x4_t acc[4][kMcNB];
 #pragma unroll
   for (int mb = 0; mb < 4; ++mb)
@@ -169,8 +178,8 @@ __global__ void __launch_bounds__(64 * kMcMaxWaves)
     if (LP > RP) continue;  // uniform
     const uint32_t lo_u = wave_search<false>(B.keys, start, end, LP);
     const uint32_t hi_u = wave_search<true>(B.keys, lo_u, end, RP);
-    for (uint32_t c0 = lo_u & ~7u; c0 < hi_u; c0 += kMcChunk) {
-      const uint32_t cnt = min(kMcChunk, hi_u - c0);
+    for (uint32_t c0 = lo_u & ~7u; c0 < hi_u; c0 += CHUNK) {
+      const uint32_t cnt = min(CHUNK, hi_u - c0);
       const uint32_t nkb = (cnt + 31) / 32;
       __syncthreads();  // the previous chunk's weight tile is consumed
       // stage the chunk; zero the weight tile's used width
@@ -185,7 +194,8 @@ __global__ void __launch_bounds__(64 * kMcMaxWaves)
         const uint32_t width8 = nkb * 4;  // 16-byte groups per row
         for (uint32_t g = tid; g < kMcCols * width8; g += nthreads) {
           const uint32_t rr = g / width8, cc = (g % width8) * 8;
-          *reinterpret_cast<uint4*>(&s_w[rr * kMcWStride + cc]) = uint4{0u, 0u, 0u, 0u};
+          *reinterpret_cast<uint4*>(&s_wh[rr * WS + cc]) = uint4{0u, 0u, 0u, 0u};
+          *reinterpret_cast<uint4*>(&s_wl[rr * WS + cc]) = uint4{0u, 0u, 0u, 0u};
         }
       }
       if (tid < 4) {
@@ -208,9 +218,12 @@ __global__ void __launch_bounds__(64 * kMcMaxWaves)
               (c2 < P.c_lo || radius_test_ref(B.xwcs, B.ywcs, B.lon_s, B.lat_s, cell, c0 + j, P.sphere_radius))) {
             double wt = pair_weight_any(P, B, kt, c2, s_exp, cell, c0 + j);
             if (B.w_s) wt *= B.w_s[c0 + j];
-            const T wr = (T)(float)wt;
-            s_w[c * kMcWStride + j] = wr;
-            wsum += (double)(float)wr;
+            // the weight as hi + lo in the value type: ~2x its significand
+            const T hi = (T)(float)wt;
+            const T lo = (T)(float)(wt - (double)(float)hi);
+            s_wh[c * WS + j] = hi;
+            s_wl[c * WS + j] = lo;
+            wsum += (double)(float)hi + (double)(float)lo;
           }
         }
       }
@@ -229,23 +242,27 @@ __global__ void __launch_bounds__(64 * kMcMaxWaves)
 #pragma unroll
           for (int mb = 0; mb < 4; ++mb) any |= (u >= mblo[mb] && u < mbhi[mb]);
           if (!any) continue;  // uniform
-          vec av[4];
-          bool cov[4];
-#pragma unroll
-          for (int mb = 0; mb < 4; ++mb) {
-            cov[mb] = u >= mblo[mb] && u < mbhi[mb];
-            if (cov[mb])
-              av[mb] = *reinterpret_cast<const vec*>(&s_w[(mb * 16 + (lane & 15)) * kMcWStride + u * 32 + koff]);
-          }
+          // B: 16 channels x 32 samples per channel block, from HBM/L2
           const T* vrow = vs + c0 + u * 32 + koff;
+          vec bv[kMcNB];
 #pragma unroll
           for (int nb = 0; nb < kMcNB; ++nb) {
             uint32_t ch = ch_base + nb * 16 + (lane & 15);
             ch = ch < nchan ? ch : nchan - 1;
-            const vec bv = *reinterpret_cast<const vec*>(vrow + (size_t)ch * n_pad);
+            bv[nb] = *reinterpret_cast<const vec*>(vrow + (size_t)ch * n_pad);
+          }
+          // A: the row block's hi and lo weights, from LDS
 #pragma unroll
-            for (int mb = 0; mb < 4; ++mb)
-              if (cov[mb]) acc[mb][nb] = Mfma<T>::op(av[mb], bv, acc[mb][nb]);
+          for (int mb = 0; mb < 4; ++mb) {
+            if (!(u >= mblo[mb] && u < mbhi[mb])) continue;
+            const uint32_t off = (mb * 16 + (lane & 15)) * WS + u * 32 + koff;
+            const vec ah = *reinterpret_cast<const vec*>(&s_wh[off]);
+            const vec al = *reinterpret_cast<const vec*>(&s_wl[off]);
+#pragma unroll
+            for (int nb = 0; nb < kMcNB; ++nb) {
+              acc[mb][nb] = Mfma<T>::op(ah, bv[nb], acc[mb][nb]);
+              acc[mb][nb] = Mfma<T>::op(al, bv[nb], acc[mb][nb]);
+            }
           }
         }
       }
@@ -383,19 +400,32 @@ extern "C" int hcg_grid_mfma(const hcg_healpix* hp, const hcg_kernel* kern, uint
                 xwcs,    ywcs,        nullptr,    nullptr};
   const uint32_t n_pad = hcg_channels_padded_len(n);
   const uint32_t groups_total = ceil_div_u32(nchan, kMcChanPerWave);  // waves needed
-  const uint32_t nw = groups_total < (uint32_t)kMcMaxWaves ? groups_total : (uint32_t)kMcMaxWaves;
+  // waves per workgroup: a power of two (each cell's nw threads are aligned
+  // consecutive lanes, reduced with xor shuffles); surplus waves only help
+  // with the weights
+  uint32_t nw = 1;
+  while (nw < groups_total && nw < (uint32_t)kMcMaxWaves) nw <<= 1;
   const uint32_t ngroups = ceil_div_u32(groups_total, nw);
   const uint32_t segs_x = ceil_div_u32(side, kMcCols);
   const uint32_t nsegs = segs_x * side;
   const dim3 grid(nsegs, ngroups);
   hipStream_t s = as_stream(stream);
-  if (dtype == HCG_DTYPE_BF16)
-    k_grid_mfma<__bf16><<<grid, 64 * nw, 0, s>>>(P, B, kern->kernel_type, static_cast<const __bf16*>(values_sorted),
-                                                  n_pad, nchan, segs_x, nsegs, cube, wsum, normalize);
-  else
-    k_grid_mfma<_Float16><<<grid, 64 * nw, 0, s>>>(P, B, kern->kernel_type,
-                                                    static_cast<const _Float16*>(values_sorted), n_pad, nchan, segs_x,
-                                                    nsegs, cube, wsum, normalize);
+  // chunk size by workgroup width: 8 waves fill a CU alone (VGPR-limited);
+  // narrower workgroups take smaller tiles so several share a CU
+  const uint32_t chunk = nw >= 8 ? 512u : (nw >= 4 ? 256u : 128u);
+#define HCG_LAUNCH_MFMA(TT, CH)                                                                               \
+  k_grid_mfma<TT, CH><<<grid, 64 * nw, 0, s>>>(P, B, kern->kernel_type, static_cast<const TT*>(values_sorted), \
+                                               n_pad, nchan, segs_x, nsegs, cube, wsum, normalize)
+  if (dtype == HCG_DTYPE_BF16) {
+    if (chunk == 512) HCG_LAUNCH_MFMA(__bf16, 512);
+    else if (chunk == 256) HCG_LAUNCH_MFMA(__bf16, 256);
+    else HCG_LAUNCH_MFMA(__bf16, 128);
+  } else {
+    if (chunk == 512) HCG_LAUNCH_MFMA(_Float16, 512);
+    else if (chunk == 256) HCG_LAUNCH_MFMA(_Float16, 256);
+    else HCG_LAUNCH_MFMA(_Float16, 128);
+  }
+#undef HCG_LAUNCH_MFMA
   HCG_CHECK_LAUNCH("k_grid_mfma");
   return HCG_OK;
 }

@@ -240,13 +240,17 @@ class CubeGridder:
              _ptr(b.lut), _ptr(b.ring_z), _ptr(b.ring_s), _ptr(b.meta), _ptr(xwcs), _ptr(ywcs), _ptr(self.cube),
              _ptr(self.wsum), 1 if normalize else 0, _stream_handle(stream))
 
+    def gather_records(self, lon, lat, stream=None):
+        """Sorted sample records and coordinates (the value slot stays 0)."""
+        b = self.plain.buf
+        call("hcg_gather_samples", self.n, _ptr(b.perm), _ptr(lon), _ptr(lat), None, capi.DTYPE_F64, None,
+             _ptr(b.records), _ptr(b.lon_s), _ptr(b.lat_s), None, None, _stream_handle(stream))
+
     def run(self, lon, lat, values, xwcs, ywcs, layout: int = 0, stream=None, normalize: bool = True):
         """One pass: keys + sort, records, LUT, channel gather, contraction."""
-        g = self.plain
-        g.presort(lon, lat, stream)
-        call("hcg_gather_samples", self.n, _ptr(g.buf.perm), _ptr(lon), _ptr(lat), None, capi.DTYPE_F64, None,
-             _ptr(g.buf.records), _ptr(g.buf.lon_s), _ptr(g.buf.lat_s), None, None, _stream_handle(stream))
-        g.ring_lut(stream)
+        self.plain.presort(lon, lat, stream)
+        self.gather_records(lon, lat, stream)
+        self.plain.ring_lut(stream)
         self.gather_channels(values, layout, stream)
         self.grid(xwcs, ywcs, normalize, stream)
 

@@ -321,11 +321,12 @@ def wcs_sin(naxis1, naxis2, crpix1, crpix2, cdelt1, cdelt2, crval1, crval2):
 
 
 def pipeline(lon, lat, data, xwcs, ywcs, side, sigma_deg, radius_deg, hpxres_deg, weights=None,
-             factor=1, kernel_type=GAUSS1D, kernel_params=None, nthreads=0):
+             factor=1, kernel_type=GAUSS1D, kernel_params=None, nthreads=0, rows=None):
     """Whole reference path on the CPU: keys, stable sort, gathers, LUT, grid.
 
     Returns a dict with keys_sorted, perm, firstring, usedrings, lut, datacube,
-    weightscube, hangs and (h, g)."""
+    weightscube, hangs and (h, g).  rows=(r0, r1) grids only that band of map
+    rows (datacube/weightscube then hold those rows; hangs is not counted)."""
     if kernel_params is None:
         kernel_params = [sigma_deg]
     h, g = prepare_grid_kernel(kernel_type, kernel_params, radius_deg, hpxres_deg, side, factor)
@@ -340,7 +341,11 @@ def pipeline(lon, lat, data, xwcs, ywcs, side, sigma_deg, radius_deg, hpxres_deg
         lut[-1] = n  # invalid keys sort last; the last ring's range ends at n
     lon_s, lat_s, data_s = lon[perm], lat[perm], data[perm]
     w_s = None if weights is None else np.asarray(weights, dtype=np.float64)[perm]
-    if used:
+    if used and rows is not None:
+        dc, wc = grid_compat_rows(h, g, lon_s, lat_s, data_s, w_s, keys, fr, used, lut, xwcs, ywcs,
+                                  rows[0], rows[1], nthreads=nthreads)
+        hangs = 0
+    elif used:
         dc, wc, hangs = grid_compat(h, g, lon_s, lat_s, data_s, w_s, keys, fr, used, lut, xwcs, ywcs,
                                     nthreads=nthreads)
     else:
