@@ -43,6 +43,8 @@ def parse():
     ap.add_argument("--cube", dest="cube", action="store_true", default=True,
                     help="also measure the multi-channel path (2M samples x 1024 BF16 channels)")
     ap.add_argument("--no-cube", dest="cube", action="store_false")
+    ap.add_argument("--calibrate-copy", dest="calibrate_copy", action="store_true",
+                    help="first run one 1 GiB device-to-device copy (a reference for HBM byte counters)")
     return ap.parse_args()
 
 
@@ -245,6 +247,15 @@ def main():
     if not torch.cuda.is_available():
         raise SystemExit("bench.py: no GPU visible")
     dev = torch.device("cuda", 0)
+
+    if args.calibrate_copy:
+        src = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("calibration copy: 1 GiB read + 1 GiB written")
+        del src, dst
 
     t0 = time.perf_counter()
     wl = synth.raster_20m() if args.workload == "raster_20m" else synth.raster_250k()

@@ -49,9 +49,34 @@ class Kernel(C.Structure):
     ]
 
 
+class MfmaShape(C.Structure):
+    _fields_ = [("tile_cols", C.c_uint32), ("waves", C.c_uint32), ("chunk", C.c_uint32),
+                ("reserved", C.c_uint32)]
+
+
 class RingMeta(C.Structure):
     _fields_ = [("firstring", C.c_uint32), ("usedrings", C.c_uint32),
                 ("n_valid", C.c_uint32), ("n", C.c_uint32)]
+
+
+class GeometryView(C.Structure):
+    _fields_ = [("n", C.c_uint32), ("side", C.c_uint32), ("bytes", C.c_size_t),
+                ("keys_sorted", C.c_void_p), ("perm", C.c_void_p), ("records", C.c_void_p),
+                ("lon_sorted", C.c_void_p), ("lat_sorted", C.c_void_p), ("lut", C.c_void_p),
+                ("ring_z", C.c_void_p), ("ring_s", C.c_void_p), ("meta", C.c_void_p),
+                ("xwcs", C.c_void_p), ("ywcs", C.c_void_p)]
+
+
+class BatchJob(C.Structure):
+    _fields_ = [("geometry", C.c_void_p), ("nchan", C.c_uint32), ("dtype", C.c_int),
+                ("values", C.c_void_p), ("cube", C.c_void_p), ("wsum", C.c_void_p),
+                ("tile_cols", C.c_uint32), ("normalize", C.c_int)]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("njobs", C.c_uint32), ("nitems", C.c_uint32), ("blocks", C.c_uint32),
+                ("waves", C.c_uint32), ("chunk", C.c_uint32), ("reserved", C.c_uint32),
+                ("device_bytes", C.c_size_t)]
 
 
 SAMPLE_RECORD_BYTES = 32  # hcg_sample: x, y, z, v doubles
@@ -95,9 +120,27 @@ PROTOTYPES = {
     "hcg_gather_channels": (_int, [_u32, _u32, _vp, _vp, _int, _int, _vp, _vp, _vp]),
     "hcg_grid_mfma": (_int, [C.POINTER(Healpix), C.POINTER(Kernel), _u32, _u32, _u32, _int, _vp, _vp, _vp, _vp,
                              _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _int, _vp]),
+    "hcg_grid_mfma_shaped": (_int, [C.POINTER(Healpix), C.POINTER(Kernel), _u32, _u32, _u32, _int, _vp, _vp, _vp,
+                                    _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _int,
+                                    C.POINTER(MfmaShape), _vp]),
     "hcg_normalize": (_int, [_u64, _vp, _vp, _vp, _vp]),
     "hcg_eval_exp_neg": (_int, [_u32, _vp, _vp, _vp]),
     "hcg_wcs_sin_p2s": (_int, [_u32, _u32, _dbl, _dbl, _dbl, _dbl, _dbl, _dbl, _dbl, _vp, _vp]),
+    "hcg_geometry_bytes": (_sz, [_u32, _u32, C.POINTER(Healpix)]),
+    "hcg_geometry_create": (_int, [C.POINTER(Healpix), C.POINTER(Kernel), _u32, _u32, C.POINTER(_vp)]),
+    "hcg_geometry_build": (_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "hcg_geometry_arrays": (_int, [_vp, C.POINTER(GeometryView)]),
+    "hcg_geometry_destroy": (_int, [_vp]),
+    "hcg_batch_job_bytes": (_sz, [_vp, _u32]),
+    "hcg_batch_create": (_int, [_u32, C.POINTER(BatchJob), _u32, _u32, C.POINTER(_vp)]),
+    "hcg_batch_enqueue": (_int, [_vp, _vp]),
+    "hcg_batch_info": (_int, [_vp, C.POINTER(BatchStats)]),
+    "hcg_batch_nonfinite": (_int, [_vp, C.POINTER(_u32), _vp]),
+    "hcg_batch_destroy": (_int, [_vp]),
+    "hcg_graph_capture_begin": (_int, [_vp]),
+    "hcg_graph_capture_end": (_int, [_vp, C.POINTER(_vp)]),
+    "hcg_graph_launch": (_int, [_vp, _vp]),
+    "hcg_graph_destroy": (_int, [_vp]),
 }
 
 _lib = None

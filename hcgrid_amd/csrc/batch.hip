@@ -1,0 +1,399 @@
+// batch.hip -- resident geometries, batched multi-channel gridding in one
+// persistent launch, and HIP graph capture of the whole sequence.
+//
+// The reference grids one map per process run (HCGrid/HCGrid.cpp:38-197;
+// HCGrid+ loops over channels of one map, HCGrid+/gridding-omp.cu:552-678).
+// Here a geometry (one map's sorted samples, ring LUT and target
+// coordinates) is built once and kept in HBM; a batch names any number of
+// (geometry, channel values, output cube) jobs of uneven sizes and grids
+// them all with one launch of k_grid_mfma_batch, whose workgroups pull
+// (job, tile, channel group) items from a device queue.  Everything a batch
+// enqueues is kernels only (no memsets, no host copies), so the sequence
+// can be captured into a graph and replayed.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "common.hpp"
+#include "mfma_job.hpp"
+
+using namespace hcg;
+
+struct hcg_geometry {
+  int device;
+  hcg_healpix hp;
+  hcg_kernel kern;
+  uint32_t n, side;
+  size_t bytes;
+  void* base;
+  uint32_t* keys;        // [n+1]
+  uint32_t* perm;        // [n]
+  hcg_sample* rec;       // [n]
+  double* lon_s;         // [n]
+  double* lat_s;         // [n]
+  uint32_t* lut;         // [nrings+2]
+  double* ring_z;        // [nrings+2]
+  double* ring_s;        // [nrings+2]
+  hcg_ring_meta* meta;   // [1]
+  double* xwcs;          // [side*side]
+  double* ywcs;          // [side*side]
+  void* ws;              // pre-sort workspace
+  size_t ws_bytes;
+};
+
+struct hcg_batch {
+  int device;
+  uint32_t njobs, nitems, waves, chunk, blocks;
+  size_t bytes;
+  void* base;
+  MfmaJob* d_jobs;
+  MfmaItem* d_items;
+  uint32_t* d_counters;  // [0] queue head, [1 + j] non-finite values of job j
+  std::vector<hcg_batch_job> jobs;
+  std::vector<void*> values_sorted;
+};
+
+namespace {
+
+constexpr size_t kAlign = 256;
+
+size_t align_up(size_t v) { return (v + kAlign - 1) & ~(kAlign - 1); }
+
+// Carves consecutive aligned regions out of one allocation.
+struct Carver {
+  char* base;
+  size_t off = 0;
+  template <typename T>
+  T* take(size_t count) {
+    T* p = base ? reinterpret_cast<T*>(base + off) : nullptr;
+    off = align_up(off + count * sizeof(T));
+    return p;
+  }
+};
+
+void geometry_layout(hcg_geometry* g, char* base) {
+  Carver c{base};
+  const size_t n = g->n, nr = (size_t)g->hp.nrings + 2, m = (size_t)g->side * g->side;
+  g->keys = c.take<uint32_t>(n + 1);
+  g->perm = c.take<uint32_t>(n ? n : 1);
+  g->rec = c.take<hcg_sample>(n ? n : 1);
+  g->lon_s = c.take<double>(n ? n : 1);
+  g->lat_s = c.take<double>(n ? n : 1);
+  g->lut = c.take<uint32_t>(nr);
+  g->ring_z = c.take<double>(nr);
+  g->ring_s = c.take<double>(nr);
+  g->meta = c.take<hcg_ring_meta>(1);
+  g->xwcs = c.take<double>(m ? m : 1);
+  g->ywcs = c.take<double>(m ? m : 1);
+  g->ws_bytes = hcg_presort_workspace_size(g->n);
+  g->ws = c.take<char>(g->ws_bytes ? g->ws_bytes : 1);
+  g->bytes = c.off;
+}
+
+__global__ void __launch_bounds__(256) k_fill_u32(uint32_t* p, uint32_t count, uint32_t v) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i < count) p[i] = v;
+}
+
+// Rough cost of one work item of a job: samples in a tile's neighbourhood
+// times (channels in the item + a fixed weight-evaluation share).
+double item_cost(const hcg_geometry* g, uint32_t tile_cols, uint32_t chans) {
+  const double side = g->side, tc = tile_cols, tr = 64.0 / tile_cols;
+  const double cell_deg = 0.01;  // only the ratio between jobs matters
+  const double d = 2.0 * g->kern.sphere_radius / cell_deg;
+  const double density = side > 0 ? (double)g->n / (side * side) : 0.0;
+  return density * (tr + d) * (tc + d) * (chans + 32.0);
+}
+
+}  // namespace
+
+extern "C" size_t hcg_geometry_bytes(uint32_t n, uint32_t side, const hcg_healpix* hp) {
+  if (!hp) return 0;
+  hcg_geometry g{};
+  g.n = n;
+  g.side = side;
+  g.hp = *hp;
+  geometry_layout(&g, nullptr);
+  return g.bytes;
+}
+
+extern "C" int hcg_geometry_create(const hcg_healpix* hp, const hcg_kernel* kern, uint32_t n, uint32_t side,
+                                   hcg_geometry** out) {
+  HCG_REQUIRE(hp && kern && out, "hcg_geometry_create: NULL argument");
+  *out = nullptr;
+  if (hp->nside == 0 || hp->nside > HCG_MAX_NSIDE || (hp->nside & (hp->nside - 1)))
+    return set_error(HCG_ERR_UNSUPPORTED, "hcg_geometry_create: nside must be a power of two <= 8192");
+  HCG_REQUIRE(n < 0xFFFFFFFFu, "hcg_geometry_create: n must be < 2^32-1");
+  HCG_REQUIRE(side <= 65535, "hcg_geometry_create: side must be <= 65535");
+  HCG_REQUIRE(kern->factor >= 1, "hcg_geometry_create: factor must be >= 1");
+  hcg_geometry* g = new (std::nothrow) hcg_geometry{};
+  if (!g) return set_error(HCG_ERR_NOMEM, "hcg_geometry_create: host allocation failed");
+  g->hp = *hp;
+  g->kern = *kern;
+  g->n = n;
+  g->side = side;
+  geometry_layout(g, nullptr);
+  hipError_t e = hipGetDevice(&g->device);
+  if (e == hipSuccess) e = hipMalloc(&g->base, g->bytes);
+  if (e != hipSuccess) {
+    delete g;
+    (void)hipGetLastError();
+    return set_error(HCG_ERR_NOMEM, std::string("hcg_geometry_create: hipMalloc: ") + hipGetErrorString(e));
+  }
+  geometry_layout(g, static_cast<char*>(g->base));
+  *out = g;
+  return HCG_OK;
+}
+
+extern "C" int hcg_geometry_destroy(hcg_geometry* g) {
+  if (!g) return HCG_OK;
+  if (g->base) HCG_CHECK_HIP(hipFree(g->base));
+  delete g;
+  return HCG_OK;
+}
+
+extern "C" int hcg_geometry_build(hcg_geometry* g, const double* lon_deg, const double* lat_deg,
+                                  const double* xwcs, const double* ywcs, void* stream) {
+  HCG_REQUIRE(g, "hcg_geometry_build: NULL geometry");
+  hipStream_t s = as_stream(stream);
+  const size_t m = (size_t)g->side * g->side;
+  if (m) {
+    HCG_REQUIRE(xwcs && ywcs, "hcg_geometry_build: NULL target coordinates");
+    if (xwcs != g->xwcs) HCG_CHECK_HIP(hipMemcpyAsync(g->xwcs, xwcs, m * 8, hipMemcpyDeviceToDevice, s));
+    if (ywcs != g->ywcs) HCG_CHECK_HIP(hipMemcpyAsync(g->ywcs, ywcs, m * 8, hipMemcpyDeviceToDevice, s));
+  }
+  int rc = hcg_presort(&g->hp, g->n, lon_deg, lat_deg, g->keys, g->perm, g->ws, g->ws_bytes, stream);
+  if (rc != HCG_OK) return rc;
+  rc = hcg_gather_samples(g->n, g->perm, lon_deg, lat_deg, nullptr, HCG_DTYPE_F64, nullptr, g->rec, g->lon_s,
+                          g->lat_s, nullptr, nullptr, stream);
+  if (rc != HCG_OK) return rc;
+  return hcg_ring_lut(&g->hp, g->n, g->keys, g->lut, g->ring_z, g->ring_s, g->meta, stream);
+}
+
+extern "C" int hcg_geometry_arrays(const hcg_geometry* g, hcg_geometry_view* v) {
+  HCG_REQUIRE(g && v, "hcg_geometry_arrays: NULL argument");
+  v->n = g->n;
+  v->side = g->side;
+  v->bytes = g->bytes;
+  v->keys_sorted = g->keys;
+  v->perm = g->perm;
+  v->records = g->rec;
+  v->lon_sorted = g->lon_s;
+  v->lat_sorted = g->lat_s;
+  v->lut = g->lut;
+  v->ring_z = g->ring_z;
+  v->ring_s = g->ring_s;
+  v->meta = g->meta;
+  v->xwcs = g->xwcs;
+  v->ywcs = g->ywcs;
+  return HCG_OK;
+}
+
+extern "C" size_t hcg_batch_job_bytes(const hcg_geometry* g, uint32_t nchan) {
+  if (!g) return 0;
+  return align_up((size_t)nchan * hcg_channels_padded_len(g->n) * 2);
+}
+
+extern "C" int hcg_batch_create(uint32_t njobs, const hcg_batch_job* jobs, uint32_t waves, uint32_t chunk,
+                                hcg_batch** out) {
+  HCG_REQUIRE(out, "hcg_batch_create: NULL out");
+  *out = nullptr;
+  HCG_REQUIRE(njobs == 0 || jobs, "hcg_batch_create: NULL jobs");
+  if (waves == 0) waves = 4;
+  if (chunk == 0) chunk = waves >= 8 ? 512 : (waves >= 4 ? 256 : 128);
+  HCG_REQUIRE(waves == 1 || waves == 2 || waves == 4 || waves == 8, "hcg_batch_create: waves must be 1, 2, 4 or 8");
+  HCG_REQUIRE(chunk == 128 || chunk == 256 || chunk == 512, "hcg_batch_create: chunk must be 128, 256 or 512");
+  for (uint32_t j = 0; j < njobs; ++j) {
+    const hcg_batch_job& b = jobs[j];
+    HCG_REQUIRE(b.geometry, "hcg_batch_create: job without geometry");
+    HCG_REQUIRE(b.dtype == HCG_DTYPE_BF16 || b.dtype == HCG_DTYPE_F16, "hcg_batch_create: dtype must be BF16 or F16");
+    HCG_REQUIRE(b.tile_cols == 0 || b.tile_cols == 8 || b.tile_cols == 16 || b.tile_cols == 32 || b.tile_cols == 64,
+                "hcg_batch_create: tile_cols must be 0, 8, 16, 32 or 64");
+    HCG_REQUIRE(b.geometry->kern.kernel_type >= HCG_GAUSS1D && b.geometry->kern.kernel_type <= HCG_TAPERED_SINC,
+                "hcg_batch_create: unknown kernel type");
+    HCG_REQUIRE(b.nchan == 0 || (b.values && b.cube), "hcg_batch_create: NULL values or cube");
+  }
+  hcg_batch* bt = new (std::nothrow) hcg_batch{};
+  if (!bt) return set_error(HCG_ERR_NOMEM, "hcg_batch_create: host allocation failed");
+  bt->njobs = njobs;
+  bt->waves = waves;
+  bt->chunk = chunk;
+  bt->jobs.assign(jobs, jobs + njobs);
+
+  // work items: jobs longest item first, each job's tiles in order, a
+  // tile's channel groups adjacent (they share the tile's samples in L2)
+  std::vector<uint32_t> order(njobs);
+  std::vector<double> cost(njobs);
+  std::vector<uint32_t> tiles(njobs), groups(njobs), tcols(njobs);
+  uint64_t nitems = 0;
+  for (uint32_t j = 0; j < njobs; ++j) {
+    const hcg_batch_job& b = jobs[j];
+    const uint32_t tc = b.tile_cols ? b.tile_cols : kMcDefaultTileCols;
+    const uint32_t side = b.geometry->side;
+    tcols[j] = tc;
+    tiles[j] = (b.nchan && side) ? ceil_div_u32(side, tc) * ceil_div_u32(side, 64 / tc) : 0;
+    groups[j] = ceil_div_u32(ceil_div_u32(b.nchan, kMcChanPerWave), waves);
+    cost[j] = item_cost(b.geometry, tc, std::min<uint32_t>(b.nchan, waves * kMcChanPerWave));
+    order[j] = j;
+    nitems += (uint64_t)tiles[j] * groups[j];
+  }
+  if (nitems >= 0xFFFFFFFFull) {
+    delete bt;
+    return set_error(HCG_ERR_INVALID, "hcg_batch_create: too many work items");
+  }
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return cost[a] > cost[b]; });
+  std::vector<MfmaItem> items;
+  items.reserve(nitems);
+  for (uint32_t j : order)
+    for (uint32_t t = 0; t < tiles[j]; ++t)
+      for (uint32_t g = 0; g < groups[j]; ++g) items.push_back(MfmaItem{j, t, g, 0});
+  bt->nitems = (uint32_t)items.size();
+
+  // one allocation: job table, items, counters, sorted channel values
+  Carver c{nullptr};
+  c.take<MfmaJob>(njobs ? njobs : 1);
+  c.take<MfmaItem>(items.empty() ? 1 : items.size());
+  c.take<uint32_t>(1 + njobs);
+  std::vector<size_t> voff(njobs);
+  for (uint32_t j = 0; j < njobs; ++j) {
+    voff[j] = c.off;
+    c.off += hcg_batch_job_bytes(jobs[j].geometry, jobs[j].nchan);
+  }
+  bt->bytes = c.off;
+  hipError_t e = hipGetDevice(&bt->device);
+  if (e == hipSuccess) e = hipMalloc(&bt->base, bt->bytes);
+  if (e != hipSuccess) {
+    delete bt;
+    (void)hipGetLastError();
+    return set_error(HCG_ERR_NOMEM, std::string("hcg_batch_create: hipMalloc: ") + hipGetErrorString(e));
+  }
+  Carver d{static_cast<char*>(bt->base)};
+  bt->d_jobs = d.take<MfmaJob>(njobs ? njobs : 1);
+  bt->d_items = d.take<MfmaItem>(items.empty() ? 1 : items.size());
+  bt->d_counters = d.take<uint32_t>(1 + njobs);
+  bt->values_sorted.resize(njobs);
+  std::vector<MfmaJob> hj(njobs);
+  for (uint32_t j = 0; j < njobs; ++j) {
+    const hcg_batch_job& b = jobs[j];
+    const hcg_geometry* g = b.geometry;
+    bt->values_sorted[j] = static_cast<char*>(bt->base) + voff[j];
+    GridBuffers B{g->rec,  g->keys, g->lon_s, g->lat_s, nullptr, g->lut, g->ring_z,
+                  g->ring_s, g->meta, g->xwcs,  g->ywcs,  nullptr, nullptr};
+    hj[j] = MfmaJob{make_grid_params(&g->hp, &g->kern, g->side),
+                    B,
+                    bt->values_sorted[j],
+                    b.cube,
+                    b.wsum,
+                    g->kern.kernel_type,
+                    hcg_channels_padded_len(g->n),
+                    b.nchan,
+                    (uint32_t)b.dtype,
+                    tile_shift_of(tcols[j]),
+                    ceil_div_u32(g->side, tcols[j]),
+                    tiles[j],
+                    (uint32_t)(b.normalize ? 1 : 0)};
+  }
+  e = hipSuccess;
+  if (njobs) e = hipMemcpy(bt->d_jobs, hj.data(), njobs * sizeof(MfmaJob), hipMemcpyHostToDevice);
+  if (e == hipSuccess && !items.empty())
+    e = hipMemcpy(bt->d_items, items.data(), items.size() * sizeof(MfmaItem), hipMemcpyHostToDevice);
+  // grid size of the persistent launch, fixed here so that enqueueing makes no queries
+  if (e == hipSuccess) {
+    uint32_t blocks = 0;
+    int rc = mfma_batch_blocks(chunk, waves, &blocks);
+    if (rc != HCG_OK) {
+      (void)hipFree(bt->base);
+      delete bt;
+      return rc;
+    }
+    bt->blocks = blocks;
+  }
+  if (e != hipSuccess) {
+    (void)hipFree(bt->base);
+    delete bt;
+    return set_error(HCG_ERR_HIP, std::string("hcg_batch_create: ") + hipGetErrorString(e));
+  }
+  *out = bt;
+  return HCG_OK;
+}
+
+extern "C" int hcg_batch_enqueue(hcg_batch* b, void* stream) {
+  HCG_REQUIRE(b, "hcg_batch_enqueue: NULL batch");
+  hipStream_t s = as_stream(stream);
+  const uint32_t nc = 1 + b->njobs;
+  k_fill_u32<<<ceil_div_u32(nc, 256), 256, 0, s>>>(b->d_counters, nc, 0u);
+  HCG_CHECK_LAUNCH("k_fill_u32");
+  for (uint32_t j = 0; j < b->njobs; ++j) {
+    const hcg_batch_job& jb = b->jobs[j];
+    if (jb.nchan == 0) continue;
+    int rc = hcg_gather_channels(jb.geometry->n, jb.nchan, jb.geometry->perm, jb.values, jb.dtype, 0,
+                                 b->values_sorted[j], b->d_counters + 1 + j, stream);
+    if (rc != HCG_OK) return rc;
+  }
+  if (b->nitems == 0) return HCG_OK;
+  return launch_mfma_batch(b->chunk, b->waves, b->blocks, b->d_jobs, b->d_items, b->nitems, b->d_counters, s);
+}
+
+extern "C" int hcg_batch_info(const hcg_batch* b, hcg_batch_stats* st) {
+  HCG_REQUIRE(b && st, "hcg_batch_info: NULL argument");
+  st->njobs = b->njobs;
+  st->nitems = b->nitems;
+  st->blocks = b->blocks;
+  st->waves = b->waves;
+  st->chunk = b->chunk;
+  st->reserved = 0;
+  st->device_bytes = b->bytes;
+  return HCG_OK;
+}
+
+extern "C" int hcg_batch_nonfinite(const hcg_batch* b, uint32_t* counts, void* stream) {
+  HCG_REQUIRE(b && counts, "hcg_batch_nonfinite: NULL argument");
+  if (b->njobs == 0) return HCG_OK;
+  hipStream_t s = as_stream(stream);
+  HCG_CHECK_HIP(hipMemcpyAsync(counts, b->d_counters + 1, b->njobs * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HCG_CHECK_HIP(hipStreamSynchronize(s));
+  return HCG_OK;
+}
+
+extern "C" int hcg_batch_destroy(hcg_batch* b) {
+  if (!b) return HCG_OK;
+  if (b->base) HCG_CHECK_HIP(hipFree(b->base));
+  delete b;
+  return HCG_OK;
+}
+
+// ---- graphs ----
+
+extern "C" int hcg_graph_capture_begin(void* stream) {
+  HCG_REQUIRE(stream, "hcg_graph_capture_begin: capture needs a created stream, not the null stream");
+  HCG_CHECK_HIP(hipStreamBeginCapture(as_stream(stream), hipStreamCaptureModeThreadLocal));
+  return HCG_OK;
+}
+
+extern "C" int hcg_graph_capture_end(void* stream, void** graph_exec) {
+  HCG_REQUIRE(stream && graph_exec, "hcg_graph_capture_end: NULL argument");
+  *graph_exec = nullptr;
+  hipGraph_t graph = nullptr;
+  HCG_CHECK_HIP(hipStreamEndCapture(as_stream(stream), &graph));
+  hipGraphExec_t exec = nullptr;
+  hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) return set_error(HCG_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  *graph_exec = exec;
+  return HCG_OK;
+}
+
+extern "C" int hcg_graph_launch(void* graph_exec, void* stream) {
+  HCG_REQUIRE(graph_exec, "hcg_graph_launch: NULL graph");
+  HCG_CHECK_HIP(hipGraphLaunch(static_cast<hipGraphExec_t>(graph_exec), as_stream(stream)));
+  return HCG_OK;
+}
+
+extern "C" int hcg_graph_destroy(void* graph_exec) {
+  if (!graph_exec) return HCG_OK;
+  HCG_CHECK_HIP(hipGraphExecDestroy(static_cast<hipGraphExec_t>(graph_exec)));
+  return HCG_OK;
+}

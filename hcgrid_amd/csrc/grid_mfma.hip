@@ -13,12 +13,11 @@
 // type's significand (relative error ~2^-17 for BF16) at twice the MFMA
 // count, which the kernel's HBM/L2 stream of V hides.
 //
-// Work decomposition: a workgroup owns one map row x 64 columns (4 MFMA
-// row blocks of 16 cells) and up to 8 x 64 channels (wave w: 4 blocks of
-// 16 channels; 4 row blocks x 4 channel blocks x 4 = 64 FP32 accumulators
-// per lane).  Wider channel ranges take more workgroups along grid.y.
-// For every non-empty ring of the
-// row segment's ring range:
+// Work decomposition: a workgroup owns a block of 64 cells (TR rows x TC
+// columns; 4 MFMA row blocks of 16 cells) and up to 8 x 64 channels (wave
+// w: 4 blocks of 16 channels; 4 row blocks x 4 channel blocks x 4 = 64 FP32
+// accumulators per lane).  Wider channel ranges take more workgroups along
+// grid.y.  For every non-empty ring of the block's ring range:
 //   1. each cell's [lpix, rpix] window (the reference's compat selection);
 //   2. the union window's sorted range by a 64-way cooperative key search;
 //   3. per chunk of <= CHUNK samples: stage unit vectors and keys in LDS,
@@ -34,6 +33,7 @@
 #include <hip/hip_runtime.h>
 
 #include "gridcore.hpp"
+#include "mfma_job.hpp"
 
 namespace hcg {
 
@@ -41,27 +41,44 @@ typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef _Float16 f16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
 
-constexpr int kMcCols = 64;                   // cells per workgroup (one map row)
-constexpr int kMcNB = 4;                      // 16-channel blocks per wave
-constexpr int kMcMaxWaves = 8;
-constexpr uint32_t kMcChanPerWave = 16 * kMcNB;
 
-template <typename T>
-struct Mfma;
-template <>
-struct Mfma<__bf16> {
-  typedef bf16x8_t vec;
-  static __device__ inline f32x4_t op(vec a, vec b, f32x4_t c) {
-    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
-  }
-};
-template <>
-struct Mfma<_Float16> {
-  typedef f16x8_t vec;
-  static __device__ inline f32x4_t op(vec a, vec b, f32x4_t c) {
-    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
-  }
-};
+typedef short s16x8_t __attribute__((ext_vector_type(8)));  // a raw 16-byte MFMA operand
+
+// Channel type of a kernel instance: fixed at compile time (one-job
+// launches) or read from the job descriptor (the batched launch, whose jobs
+// may mix BF16 and FP16).  Values and weights are kept as raw 16-bit words;
+// the type matters only where a weight is rounded and where the MFMA issues.
+enum : int { kDtBf16 = 0, kDtF16 = 1, kDtAny = 2 };
+
+template <int DT>
+__device__ inline bool dt_is_bf16(uint32_t dtype) {
+  if constexpr (DT == kDtBf16) return true;
+  else if constexpr (DT == kDtF16) return false;
+  else return dtype == HCG_DTYPE_BF16;
+}
+
+__device__ inline f32x4_t mfma16(bool bf, s16x8_t a, s16x8_t b, f32x4_t c) {
+  if (bf)
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, a), __builtin_bit_cast(bf16x8_t, b),
+                                                   c, 0, 0, 0);
+  return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8_t, a), __builtin_bit_cast(f16x8_t, b), c,
+                                                0, 0, 0);
+}
+
+__device__ inline uint16_t round16(float f, bool bf) {
+  return bf ? __builtin_bit_cast(uint16_t, (__bf16)f) : __builtin_bit_cast(uint16_t, (_Float16)f);
+}
+
+__device__ inline float widen16(uint16_t h, bool bf) {
+  return bf ? (float)__builtin_bit_cast(__bf16, h) : (float)__builtin_bit_cast(_Float16, h);
+}
+
+// One k-block of B for kMcNB channel blocks: 16 bytes per lane per block
+// from row ch of the sorted channel-major values (row length n_pad).
+__device__ inline void load_b_rows(s16x8_t* b, const uint16_t* p, const uint32_t* ch, uint32_t n_pad) {
+#pragma unroll
+  for (int nb = 0; nb < kMcNB; ++nb) b[nb] = *reinterpret_cast<const s16x8_t*>(p + (size_t)ch[nb] * n_pad);
+}
 
 // FP64 weight of a selected pair (the single-channel kernel's evaluation).
 __device__ inline double pair_weight_any(const GridParams& P, const GridBuffers& B, uint32_t kt, double c2,
@@ -74,24 +91,50 @@ __device__ inline double pair_weight_any(const GridParams& P, const GridBuffers&
                                           B.lat_s, 0.25 * c2, cell, i);
 }
 
+// LDS of one workgroup (the weight tiles as raw 16-bit storage).
+template <uint32_t CHUNK>
+struct MfmaLds {
+  __attribute__((aligned(16))) uint16_t wh[kMcCols * (CHUNK + 8)];
+  __attribute__((aligned(16))) uint16_t wl[kMcCols * (CHUNK + 8)];
+  double x[CHUNK], y[CHUNK], z[CHUNK];
+  uint32_t key[CHUNK];
+  uint32_t mb[4][2];
+  uint32_t red[kMcMaxWaves][2];
+  double wsum[kMcCols];
+};
+
 // CHUNK = staged samples per chunk.  LDS: two weight tiles of 64 x (CHUNK+8)
 // elements (16-byte row padding) + CHUNK unit vectors and keys: 148 KB at
 // 512 (one 8-wave workgroup per CU), 75 KB at 256, 38 KB at 128.
-template <typename T, uint32_t CHUNK>
-__global__ void __launch_bounds__(64 * kMcMaxWaves)
-    k_grid_mfma(GridParams P, GridBuffers B, uint32_t kt, const T* __restrict__ vs, uint32_t n_pad, uint32_t nchan,
-                uint32_t segs_x, uint32_t nsegs, float* __restrict__ cube, float* __restrict__ wsum_out,
-                int normalize) {
-  typedef typename Mfma<T>::vec vec;
+//
+// The 64 cells of a workgroup form a TR x TC block of the map (TC = 1 <<
+// tc_shift columns, TR = 64 / TC rows); cell c of the block is (c / TC,
+// c % TC) and MFMA row block mb holds cells 16 mb .. 16 mb + 15.  Square-ish
+// blocks share more of their neighbourhood, so each channel value is
+// re-read by fewer workgroups: (TR + d)(TC + d) / 64 reads per value for a
+// support of d cells, 16.8 for 1 x 64 and 7.7 for 4 x 16 at d = 13.
+template <int DT, uint32_t CHUNK>
+__device__ __forceinline__ void mfma_block(const MfmaJob& J, uint32_t tile, uint32_t group, MfmaLds<CHUNK>& L,
+                                           const double* s_exp) {
+  typedef s16x8_t vec;
   constexpr uint32_t WS = CHUNK + 8;  // weight-tile row stride (elements)
-  __shared__ __attribute__((aligned(16))) T s_wh[kMcCols * WS];
-  __shared__ __attribute__((aligned(16))) T s_wl[kMcCols * WS];
-  __shared__ double s_x[CHUNK], s_y[CHUNK], s_z[CHUNK];
-  __shared__ uint32_t s_key[CHUNK];
-  __shared__ double s_exp[64];
-  __shared__ uint32_t s_mb[4][2];
-  __shared__ uint32_t s_red[kMcMaxWaves][2];
-  __shared__ double s_wsum[kMcCols];
+  uint16_t* s_wh = L.wh;
+  uint16_t* s_wl = L.wl;
+  const bool bf = dt_is_bf16<DT>(J.dtype);
+  double* s_x = L.x;
+  double* s_y = L.y;
+  double* s_z = L.z;
+  uint32_t* s_key = L.key;
+  auto& s_mb = L.mb;
+  auto& s_red = L.red;
+  double* s_wsum = L.wsum;
+  const GridParams& P = J.P;
+  const GridBuffers& B = J.B;
+  const uint16_t* __restrict__ vs = static_cast<const uint16_t*>(J.vs);
+  const uint32_t kt = J.kt, n_pad = J.n_pad, nchan = J.nchan, tc_shift = J.tc_shift, tiles_x = J.tiles_x;
+  float* __restrict__ cube = J.cube;
+  float* __restrict__ wsum_out = J.wsum;
+  const int normalize = J.normalize;
 
   const hcg_ring_meta m = *B.meta;
   const uint32_t nthreads = blockDim.x;
@@ -100,18 +143,16 @@ __global__ void __launch_bounds__(64 * kMcMaxWaves)
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = __lane_id();
   const uint32_t wave = tid >> 6;
-  if (wave == 0) exp_table_load(s_exp);
 
-  // XCD-contiguous runs of row segments (block b runs on XCD b % 8)
-  uint32_t seg = blockIdx.x;
-  if ((nsegs & 7u) == 0) seg = (blockIdx.x & 7u) * (nsegs >> 3) + (blockIdx.x >> 3);
-  const uint32_t row = seg / segs_x;
-  const uint32_t x0 = (seg % segs_x) * kMcCols;
+  const uint32_t tc_mask = (1u << tc_shift) - 1u;
+  const uint32_t y0 = (tile / tiles_x) << (6u - tc_shift);
+  const uint32_t x0 = (tile % tiles_x) << tc_shift;
   const uint32_t M = P.side * P.side;
 
   // this thread's cell: c = tid / S, sub = tid % S
   const uint32_t c = tid / S, sub = tid % S;
-  const uint32_t col = x0 + c;
+  const uint32_t row = y0 + (c >> tc_shift);
+  const uint32_t col = x0 + (c & tc_mask);
   const bool live = col < P.side && row < P.side;
   const uint32_t cell = live ? row * P.side + col : 0;
   const uint32_t anchor = live ? row * P.side + (col / P.factor) * P.factor : 0;
@@ -132,7 +173,16 @@ __global__ void __launch_bounds__(64 * kMcMaxWaves)
   for (int mb = 0; mb < 4; ++mb)
 #pragma unroll
     for (int nb = 0; nb < kMcNB; ++nb) acc[mb][nb] = f32x4_t{0.f, 0.f, 0.f, 0.f};
-  const uint32_t ch_base = (blockIdx.y * nw + wave) * kMcChanPerWave;
+  const uint32_t ch_base = (group * nw + wave) * kMcChanPerWave;
+  // B operand addressing: lane l reads samples 8 (l / 16) .. +7 of a
+  // k-block for channel ch_base + 16 nb + l % 16 (clamped to the last one)
+  const uint32_t koff = (lane >> 4) * 8;
+  uint32_t bofs[kMcNB];
+#pragma unroll
+  for (int nb = 0; nb < kMcNB; ++nb) {
+    const uint32_t ch = ch_base + nb * 16 + (lane & 15);
+    bofs[nb] = (ch < nchan ? ch : nchan - 1);
+  }
 
   // ring span of the row segment
   {
@@ -203,6 +253,10 @@ __global__ void __launch_bounds__(64 * kMcMaxWaves)
         s_mb[tid][1] = 0u;
       }
       __syncthreads();
+      // channel values of the chunk's first k-block, in flight while the
+      // weights are computed
+      vec bn[kMcNB];
+      if (ch_base < nchan) load_b_rows(bn, vs + c0 + koff, bofs, n_pad);
       // weights of this thread's cell over its band in the chunk
       if (act) {
         const uint32_t ja = lower_bound_u32(s_key, 0, cnt, lp);
@@ -219,11 +273,12 @@ __global__ void __launch_bounds__(64 * kMcMaxWaves)
             double wt = pair_weight_any(P, B, kt, c2, s_exp, cell, c0 + j);
             if (B.w_s) wt *= B.w_s[c0 + j];
             // the weight as hi + lo in the value type: ~2x its significand
-            const T hi = (T)(float)wt;
-            const T lo = (T)(float)(wt - (double)(float)hi);
+            const uint16_t hi = round16((float)wt, bf);
+            const float hif = widen16(hi, bf);
+            const uint16_t lo = round16((float)(wt - (double)hif), bf);
             s_wh[c * WS + j] = hi;
             s_wl[c * WS + j] = lo;
-            wsum += (double)(float)hi + (double)(float)lo;
+            wsum += (double)hif + (double)widen16(lo, bf);
           }
         }
       }
@@ -236,21 +291,13 @@ __global__ void __launch_bounds__(64 * kMcMaxWaves)
         mbhi[mb] = s_mb[mb][1];
       }
       if (ch_base < nchan) {
-        const uint32_t koff = (lane >> 4) * 8;
+        // software pipeline: block u's MFMAs run while block u+1's channel
+        // values load
         for (uint32_t u = 0; u < nkb; ++u) {
-          bool any = false;
+          vec bc[kMcNB];
 #pragma unroll
-          for (int mb = 0; mb < 4; ++mb) any |= (u >= mblo[mb] && u < mbhi[mb]);
-          if (!any) continue;  // uniform
-          // B: 16 channels x 32 samples per channel block, from HBM/L2
-          const T* vrow = vs + c0 + u * 32 + koff;
-          vec bv[kMcNB];
-#pragma unroll
-          for (int nb = 0; nb < kMcNB; ++nb) {
-            uint32_t ch = ch_base + nb * 16 + (lane & 15);
-            ch = ch < nchan ? ch : nchan - 1;
-            bv[nb] = *reinterpret_cast<const vec*>(vrow + (size_t)ch * n_pad);
-          }
+          for (int nb = 0; nb < kMcNB; ++nb) bc[nb] = bn[nb];
+          if (u + 1 < nkb) load_b_rows(bn, vs + c0 + (u + 1) * 32 + koff, bofs, n_pad);
           // A: the row block's hi and lo weights, from LDS
 #pragma unroll
           for (int mb = 0; mb < 4; ++mb) {
@@ -260,8 +307,8 @@ __global__ void __launch_bounds__(64 * kMcMaxWaves)
             const vec al = *reinterpret_cast<const vec*>(&s_wl[off]);
 #pragma unroll
             for (int nb = 0; nb < kMcNB; ++nb) {
-              acc[mb][nb] = Mfma<T>::op(ah, bv[nb], acc[mb][nb]);
-              acc[mb][nb] = Mfma<T>::op(al, bv[nb], acc[mb][nb]);
+              acc[mb][nb] = mfma16(bf, ah, bc[nb], acc[mb][nb]);
+              acc[mb][nb] = mfma16(bf, al, bc[nb], acc[mb][nb]);
             }
           }
         }
@@ -273,8 +320,8 @@ __global__ void __launch_bounds__(64 * kMcMaxWaves)
   for (uint32_t off = 1; off < S; off <<= 1) wsum += __shfl_xor(wsum, off, 64);
   if (sub == 0) s_wsum[c] = wsum;
   __syncthreads();
-  if (blockIdx.y == 0 && wsum_out && sub == 0 && live) wsum_out[cell] = (float)wsum;
-  if (ch_base < nchan && row < P.side) {
+  if (group == 0 && wsum_out && sub == 0 && live) wsum_out[cell] = (float)wsum;
+  if (ch_base < nchan) {
 #pragma unroll
     for (int mb = 0; mb < 4; ++mb) {
 #pragma unroll
@@ -284,15 +331,88 @@ __global__ void __launch_bounds__(64 * kMcMaxWaves)
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
           const uint32_t cl = mb * 16 + (lane >> 4) * 4 + j;
-          const uint32_t colj = x0 + cl;
-          if (colj >= P.side) continue;
+          const uint32_t rowj = y0 + (cl >> tc_shift), colj = x0 + (cl & tc_mask);
+          if (colj >= P.side || rowj >= P.side) continue;
           const double v = (double)acc[mb][nb][j];
           const float o = normalize ? (float)(v / s_wsum[cl]) : (float)v;
-          cube[(size_t)ch * M + (size_t)row * P.side + colj] = o;
+          cube[(size_t)ch * M + (size_t)rowj * P.side + colj] = o;
         }
       }
     }
   }
+}
+
+
+// One job, one launch: grid = (tiles, channel groups), tiles in XCD-contiguous
+// runs (block b runs on XCD b % 8, so neighbouring tiles share an L2).
+template <int DT, uint32_t CHUNK>
+__global__ void __launch_bounds__(64 * kMcMaxWaves) k_grid_mfma(MfmaJob J) {
+  __shared__ MfmaLds<CHUNK> L;
+  __shared__ double s_exp[64];
+  if (threadIdx.x < 64) exp_table_load(s_exp);
+  uint32_t tile = blockIdx.x;
+  const uint32_t ntiles = J.ntiles;
+  if ((ntiles & 7u) == 0) tile = (blockIdx.x & 7u) * (ntiles >> 3) + (blockIdx.x >> 3);
+  mfma_block<DT, CHUNK>(J, tile, blockIdx.y, L, s_exp);
+}
+
+// Many jobs, one launch: a persistent grid pulls work items (job, tile,
+// channel group) from a device queue in list order until the list is
+// exhausted; every workgroup exits when its pull passes nitems.  The list
+// is ordered longest-first by the host, so the tail holds the short items.
+template <uint32_t CHUNK>
+__global__ void __launch_bounds__(64 * kMcMaxWaves)
+    k_grid_mfma_batch(const MfmaJob* __restrict__ jobs, const MfmaItem* __restrict__ items, uint32_t nitems,
+                      uint32_t* __restrict__ head) {
+  __shared__ MfmaLds<CHUNK> L;
+  __shared__ double s_exp[64];
+  __shared__ uint32_t s_item;
+  if (threadIdx.x < 64) exp_table_load(s_exp);
+  for (;;) {
+    if (threadIdx.x == 0) s_item = atomicAdd(head, 1u);
+    __syncthreads();
+    const uint32_t it = __builtin_amdgcn_readfirstlane(s_item);
+    if (it >= nitems) break;  // uniform: every thread read the same value
+    const uint32_t jb = __builtin_amdgcn_readfirstlane(items[it].job);
+    const uint32_t tile = __builtin_amdgcn_readfirstlane(items[it].tile);
+    const uint32_t group = __builtin_amdgcn_readfirstlane(items[it].group);
+    mfma_block<kDtAny, CHUNK>(jobs[jb], tile, group, L, s_exp);
+    __syncthreads();  // LDS and s_item are free for the next item
+  }
+}
+
+int mfma_batch_blocks(uint32_t chunk, uint32_t waves, uint32_t* blocks) {
+  if (!(waves == 1 || waves == 2 || waves == 4 || waves == 8))
+    return set_error(HCG_ERR_INVALID, "batch: waves must be 1, 2, 4 or 8");
+  if (!(chunk == 128 || chunk == 256 || chunk == 512))
+    return set_error(HCG_ERR_INVALID, "batch: chunk must be 128, 256 or 512");
+  const void* fn = chunk == 512 ? reinterpret_cast<const void*>(&k_grid_mfma_batch<512>)
+                   : chunk == 256 ? reinterpret_cast<const void*>(&k_grid_mfma_batch<256>)
+                                  : reinterpret_cast<const void*>(&k_grid_mfma_batch<128>);
+  int dev = 0, cus = 0, per_cu = 0;
+  HCG_CHECK_HIP(hipGetDevice(&dev));
+  HCG_CHECK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  HCG_CHECK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 64 * waves, 0));
+  if (per_cu < 1) per_cu = 1;
+  *blocks = (uint32_t)(cus * per_cu);
+  return HCG_OK;
+}
+
+int launch_mfma_batch(uint32_t chunk, uint32_t waves, uint32_t blocks, const MfmaJob* jobs, const MfmaItem* items,
+                      uint32_t nitems, uint32_t* head, hipStream_t s) {
+  if (!(waves == 1 || waves == 2 || waves == 4 || waves == 8))
+    return set_error(HCG_ERR_INVALID, "batch: waves must be 1, 2, 4 or 8");
+  if (!(chunk == 128 || chunk == 256 || chunk == 512))
+    return set_error(HCG_ERR_INVALID, "batch: chunk must be 128, 256 or 512");
+  if (blocks == 0) return set_error(HCG_ERR_INVALID, "batch: zero blocks");
+  if (chunk == 512)
+    k_grid_mfma_batch<512><<<blocks, 64 * waves, 0, s>>>(jobs, items, nitems, head);
+  else if (chunk == 256)
+    k_grid_mfma_batch<256><<<blocks, 64 * waves, 0, s>>>(jobs, items, nitems, head);
+  else
+    k_grid_mfma_batch<128><<<blocks, 64 * waves, 0, s>>>(jobs, items, nitems, head);
+  HCG_CHECK_LAUNCH("k_grid_mfma_batch");
+  return HCG_OK;
 }
 
 // Channel gather: sorted, channel-major, zero-padded to n_pad.
@@ -376,12 +496,13 @@ extern "C" int hcg_gather_channels(uint32_t n, uint32_t nchan, const uint32_t* p
   return HCG_OK;
 }
 
-extern "C" int hcg_grid_mfma(const hcg_healpix* hp, const hcg_kernel* kern, uint32_t side, uint32_t n, uint32_t nchan,
-                             int dtype, const hcg_sample* records, const uint32_t* keys_sorted,
-                             const double* lon_sorted, const double* lat_sorted, const double* weights_sorted,
-                             const void* values_sorted, const uint32_t* lut, const double* ring_z,
-                             const double* ring_s, const hcg_ring_meta* meta, const double* xwcs, const double* ywcs,
-                             float* cube, float* wsum, int normalize, void* stream) {
+extern "C" int hcg_grid_mfma_shaped(const hcg_healpix* hp, const hcg_kernel* kern, uint32_t side, uint32_t n,
+                                    uint32_t nchan, int dtype, const hcg_sample* records, const uint32_t* keys_sorted,
+                                    const double* lon_sorted, const double* lat_sorted, const double* weights_sorted,
+                                    const void* values_sorted, const uint32_t* lut, const double* ring_z,
+                                    const double* ring_s, const hcg_ring_meta* meta, const double* xwcs,
+                                    const double* ywcs, float* cube, float* wsum, int normalize,
+                                    const hcg_mfma_shape* shape, void* stream) {
   HCG_REQUIRE(hp && kern, "hcg_grid_mfma: NULL hp/kern");
   if (hp->nside == 0 || hp->nside > HCG_MAX_NSIDE)
     return set_error(HCG_ERR_UNSUPPORTED, "hcg_grid_mfma: nside must be <= 8192");
@@ -390,11 +511,18 @@ extern "C" int hcg_grid_mfma(const hcg_healpix* hp, const hcg_kernel* kern, uint
   HCG_REQUIRE(kern->factor >= 1, "hcg_grid_mfma: factor must be >= 1");
   HCG_REQUIRE(dtype == HCG_DTYPE_BF16 || dtype == HCG_DTYPE_F16, "hcg_grid_mfma: dtype must be BF16 or F16");
   HCG_REQUIRE(side <= 65535, "hcg_grid_mfma: side must be <= 65535");
+  hcg_mfma_shape sh = shape ? *shape : hcg_mfma_shape{0, 0, 0, 0};
+  HCG_REQUIRE(sh.tile_cols == 0 || sh.tile_cols == 8 || sh.tile_cols == 16 || sh.tile_cols == 32 ||
+                  sh.tile_cols == 64,
+              "hcg_grid_mfma: shape.tile_cols must be 0, 8, 16, 32 or 64");
+  HCG_REQUIRE(sh.waves == 0 || sh.waves == 1 || sh.waves == 2 || sh.waves == 4 || sh.waves == 8,
+              "hcg_grid_mfma: shape.waves must be 0, 1, 2, 4 or 8");
+  HCG_REQUIRE(sh.chunk == 0 || sh.chunk == 128 || sh.chunk == 256 || sh.chunk == 512,
+              "hcg_grid_mfma: shape.chunk must be 0, 128, 256 or 512");
   if (side == 0 || nchan == 0) return HCG_OK;
   HCG_REQUIRE(records && keys_sorted && lon_sorted && lat_sorted && values_sorted && lut && ring_z && ring_s && meta &&
                   xwcs && ywcs && cube,
               "hcg_grid_mfma: NULL buffer");
-  (void)n;
   GridParams P = make_grid_params(hp, kern, side);
   GridBuffers B{records, keys_sorted, lon_sorted, lat_sorted, weights_sorted, lut, ring_z, ring_s, meta,
                 xwcs,    ywcs,        nullptr,    nullptr};
@@ -403,29 +531,46 @@ extern "C" int hcg_grid_mfma(const hcg_healpix* hp, const hcg_kernel* kern, uint
   // waves per workgroup: a power of two (each cell's nw threads are aligned
   // consecutive lanes, reduced with xor shuffles); surplus waves only help
   // with the weights
-  uint32_t nw = 1;
-  while (nw < groups_total && nw < (uint32_t)kMcMaxWaves) nw <<= 1;
+  uint32_t nw = sh.waves;
+  if (nw == 0) {
+    nw = 1;
+    while (nw < groups_total && nw < (uint32_t)kMcMaxWaves) nw <<= 1;
+  }
   const uint32_t ngroups = ceil_div_u32(groups_total, nw);
-  const uint32_t segs_x = ceil_div_u32(side, kMcCols);
-  const uint32_t nsegs = segs_x * side;
-  const dim3 grid(nsegs, ngroups);
+  const uint32_t tc = sh.tile_cols ? sh.tile_cols : kMcDefaultTileCols;
+  const uint32_t tc_shift = tc == 8 ? 3u : tc == 16 ? 4u : tc == 32 ? 5u : 6u;
+  const uint32_t tr = 64u / tc;
+  const uint32_t tiles_x = ceil_div_u32(side, tc);
+  const uint32_t ntiles = tiles_x * ceil_div_u32(side, tr);
+  const dim3 grid(ntiles, ngroups);
   hipStream_t s = as_stream(stream);
   // chunk size by workgroup width: 8 waves fill a CU alone (VGPR-limited);
   // narrower workgroups take smaller tiles so several share a CU
-  const uint32_t chunk = nw >= 8 ? 512u : (nw >= 4 ? 256u : 128u);
-#define HCG_LAUNCH_MFMA(TT, CH)                                                                               \
-  k_grid_mfma<TT, CH><<<grid, 64 * nw, 0, s>>>(P, B, kern->kernel_type, static_cast<const TT*>(values_sorted), \
-                                               n_pad, nchan, segs_x, nsegs, cube, wsum, normalize)
+  const uint32_t chunk = sh.chunk ? sh.chunk : (nw >= 8 ? 512u : (nw >= 4 ? 256u : 128u));
+  MfmaJob J{P, B, values_sorted, cube, wsum, kern->kernel_type, n_pad, nchan, (uint32_t)dtype,
+            tc_shift, tiles_x, ntiles, (uint32_t)(normalize ? 1 : 0)};
+#define HCG_LAUNCH_MFMA(TT, CH) k_grid_mfma<TT, CH><<<grid, 64 * nw, 0, s>>>(J)
   if (dtype == HCG_DTYPE_BF16) {
-    if (chunk == 512) HCG_LAUNCH_MFMA(__bf16, 512);
-    else if (chunk == 256) HCG_LAUNCH_MFMA(__bf16, 256);
-    else HCG_LAUNCH_MFMA(__bf16, 128);
+    if (chunk == 512) HCG_LAUNCH_MFMA(kDtBf16, 512);
+    else if (chunk == 256) HCG_LAUNCH_MFMA(kDtBf16, 256);
+    else HCG_LAUNCH_MFMA(kDtBf16, 128);
   } else {
-    if (chunk == 512) HCG_LAUNCH_MFMA(_Float16, 512);
-    else if (chunk == 256) HCG_LAUNCH_MFMA(_Float16, 256);
-    else HCG_LAUNCH_MFMA(_Float16, 128);
+    if (chunk == 512) HCG_LAUNCH_MFMA(kDtF16, 512);
+    else if (chunk == 256) HCG_LAUNCH_MFMA(kDtF16, 256);
+    else HCG_LAUNCH_MFMA(kDtF16, 128);
   }
 #undef HCG_LAUNCH_MFMA
   HCG_CHECK_LAUNCH("k_grid_mfma");
   return HCG_OK;
+}
+
+extern "C" int hcg_grid_mfma(const hcg_healpix* hp, const hcg_kernel* kern, uint32_t side, uint32_t n, uint32_t nchan,
+                             int dtype, const hcg_sample* records, const uint32_t* keys_sorted,
+                             const double* lon_sorted, const double* lat_sorted, const double* weights_sorted,
+                             const void* values_sorted, const uint32_t* lut, const double* ring_z,
+                             const double* ring_s, const hcg_ring_meta* meta, const double* xwcs, const double* ywcs,
+                             float* cube, float* wsum, int normalize, void* stream) {
+  return hcg_grid_mfma_shaped(hp, kern, side, n, nchan, dtype, records, keys_sorted, lon_sorted, lat_sorted,
+                              weights_sorted, values_sorted, lut, ring_z, ring_s, meta, xwcs, ywcs, cube, wsum,
+                              normalize, nullptr, stream);
 }

@@ -91,7 +91,7 @@ __device__ inline bool ring_window(const GridParams& P, const Window& w, uint64_
 
 // The reference's radius test (HCGrid/helpers.cu:128-130, gridding.cu:381-386)
 // for a pair inside the guard band.
-static __device__ __noinline__ bool radius_test_ref(const double* xwcs, const double* ywcs, const double* lon_s,
+[[maybe_unused]] static __device__ __noinline__ bool radius_test_ref(const double* xwcs, const double* ywcs, const double* lon_s,
                                              const double* lat_s, uint32_t cell, uint32_t i, double radius) {
   const double l1 = xwcs[cell] * kDeg2Rad, b1 = ywcs[cell] * kDeg2Rad;
   const double l2 = lon_s[i] * kDeg2Rad, b2 = lat_s[i] * kDeg2Rad;
@@ -101,7 +101,7 @@ static __device__ __noinline__ bool radius_test_ref(const double* xwcs, const do
   return d < radius;
 }
 
-static __device__ __noinline__ double asin2_direct(double h) {
+[[maybe_unused]] static __device__ __noinline__ double asin2_direct(double h) {
   const double a = asin(sqrt(h));
   return a * a;
 }
@@ -125,7 +125,7 @@ __device__ inline double gauss_exponent_scaled(const GridParams& P, double c2) {
 
 // GAUSS2D / TAPERED_SINC weights (HCGrid/gridding.cu:242-256).
 template <int KT>
-static __device__ __noinline__ double weight_general(double p0, double p1, double p2, const double* xwcs,
+[[maybe_unused]] static __device__ __noinline__ double weight_general(double p0, double p1, double p2, const double* xwcs,
                                               const double* ywcs, const double* lon_s, const double* lat_s,
                                               double h, uint32_t cell, uint32_t i) {
   const double sdist = 2.0 * sqrt(asin2_of_h(h)) * kRad2Deg;

@@ -1,0 +1,46 @@
+// mfma_job.hpp -- the multi-channel job descriptor shared by the one-job
+// launch (hcg_grid_mfma) and the persistent batched launch (hcg_batch_*).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "gridcore.hpp"
+
+namespace hcg {
+
+constexpr int kMcCols = 64;                   // cells per workgroup
+constexpr uint32_t kMcDefaultTileCols = 16;   // default block: 4 rows x 16 columns
+constexpr int kMcNB = 4;                      // 16-channel blocks per wave
+constexpr int kMcMaxWaves = 8;
+constexpr uint32_t kMcChanPerWave = 16 * kMcNB;
+
+// One multi-channel job: geometry, sorted channel values and outputs, with
+// its launch shape (tiles of (64 >> tc_shift) x (1 << tc_shift) cells).
+struct MfmaJob {
+  GridParams P;
+  GridBuffers B;
+  const void* vs;     // sorted channel values [nchan][n_pad], BF16 or FP16
+  float* cube;        // [nchan][side * side]
+  float* wsum;        // [side * side] or NULL
+  uint32_t kt, n_pad, nchan, dtype;
+  uint32_t tc_shift, tiles_x, ntiles, normalize;
+};
+
+// One unit of the persistent launch: a tile of one job, one channel group.
+struct MfmaItem {
+  uint32_t job, tile, group, pad;
+};
+
+inline uint32_t tile_shift_of(uint32_t tile_cols) {
+  return tile_cols == 8 ? 3u : tile_cols == 16 ? 4u : tile_cols == 32 ? 5u : 6u;
+}
+
+// Grid size of the persistent kernel: CUs x resident workgroups per CU.
+int mfma_batch_blocks(uint32_t chunk, uint32_t waves, uint32_t* blocks);
+
+// Launches the persistent kernel with `blocks` workgroups of `waves` waves
+// over `nitems` work items (head: a device counter that reads 0 at launch).
+int launch_mfma_batch(uint32_t chunk, uint32_t waves, uint32_t blocks, const MfmaJob* jobs, const MfmaItem* items,
+                      uint32_t nitems, uint32_t* head, hipStream_t s);
+
+}  // namespace hcg

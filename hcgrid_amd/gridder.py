@@ -233,12 +233,14 @@ class CubeGridder:
         call("hcg_gather_channels", self.n, self.nchan, _ptr(self.plain.buf.perm), _ptr(values), self.dcode, layout,
              _ptr(self.values_sorted), _ptr(self.nonfinite), _stream_handle(stream))
 
-    def grid(self, xwcs, ywcs, normalize: bool = True, stream=None):
+    def grid(self, xwcs, ywcs, normalize: bool = True, stream=None, shape=None):
+        """shape: None (library defaults) or (tile_cols, waves, chunk), 0 = default."""
         b = self.plain.buf
-        call("hcg_grid_mfma", C.byref(self.hp), C.byref(self.kern), self.side, self.n, self.nchan, self.dcode,
-             _ptr(b.records), _ptr(b.keys), _ptr(b.lon_s), _ptr(b.lat_s), None, _ptr(self.values_sorted),
-             _ptr(b.lut), _ptr(b.ring_z), _ptr(b.ring_s), _ptr(b.meta), _ptr(xwcs), _ptr(ywcs), _ptr(self.cube),
-             _ptr(self.wsum), 1 if normalize else 0, _stream_handle(stream))
+        sh = None if shape is None else C.byref(capi.MfmaShape(*shape, 0))
+        call("hcg_grid_mfma_shaped", C.byref(self.hp), C.byref(self.kern), self.side, self.n, self.nchan,
+             self.dcode, _ptr(b.records), _ptr(b.keys), _ptr(b.lon_s), _ptr(b.lat_s), None,
+             _ptr(self.values_sorted), _ptr(b.lut), _ptr(b.ring_z), _ptr(b.ring_s), _ptr(b.meta), _ptr(xwcs),
+             _ptr(ywcs), _ptr(self.cube), _ptr(self.wsum), 1 if normalize else 0, sh, _stream_handle(stream))
 
     def gather_records(self, lon, lat, stream=None):
         """Sorted sample records and coordinates (the value slot stays 0)."""
@@ -246,25 +248,27 @@ class CubeGridder:
         call("hcg_gather_samples", self.n, _ptr(b.perm), _ptr(lon), _ptr(lat), None, capi.DTYPE_F64, None,
              _ptr(b.records), _ptr(b.lon_s), _ptr(b.lat_s), None, None, _stream_handle(stream))
 
-    def run(self, lon, lat, values, xwcs, ywcs, layout: int = 0, stream=None, normalize: bool = True):
+    def run(self, lon, lat, values, xwcs, ywcs, layout: int = 0, stream=None, normalize: bool = True,
+            shape=None):
         """One pass: keys + sort, records, LUT, channel gather, contraction."""
         self.plain.presort(lon, lat, stream)
         self.gather_records(lon, lat, stream)
         self.plain.ring_lut(stream)
         self.gather_channels(values, layout, stream)
-        self.grid(xwcs, ywcs, normalize, stream)
+        self.grid(xwcs, ywcs, normalize, stream, shape)
 
 
 def grid_cube(lon, lat, values_cm, xwcs, ywcs, side: int, sigma: float, radius: float, hpx_res: float,
-              dtype: str = "bf16", layout: int = 0, device: int = 0) -> dict:
-    """Host arrays in/out: multi-channel pass (values_cm: torch tensor on the device)."""
+              dtype: str = "bf16", layout: int = 0, device: int = 0, shape=None) -> dict:
+    """Host arrays in/out: multi-channel pass (values_cm: torch tensor on the device).
+    shape: None or (tile_cols, waves, chunk) for hcg_grid_mfma_shaped."""
     torch = _torch()
     kern, hp = prepare_grid_kernel(GAUSS1D, [sigma], radius, hpx_res, 1)
     nchan = values_cm.shape[0] if layout == 0 else values_cm.shape[1]
     g = CubeGridder(len(lon), side, nchan, kern, hp, dtype=dtype, device=device)
     dev = g.plain.device
     t = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device=dev)  # noqa: E731
-    g.run(t(lon), t(lat), values_cm.contiguous(), t(xwcs), t(ywcs), layout=layout)
+    g.run(t(lon), t(lat), values_cm.contiguous(), t(xwcs), t(ywcs), layout=layout, shape=shape)
     torch.cuda.synchronize()
     return dict(cube=g.cube.float().cpu().numpy().copy(), wsum=g.wsum.cpu().numpy().copy(),
                 perm=g.plain.perm(), nonfinite=int(g.nonfinite.item()), meta=g.plain.meta())

@@ -177,3 +177,101 @@ def raster_cube(n_lines: int = 500, n_per_line: int = 4000, dec0: float = 25.01,
     ra_c = ra0 + dra * (n_per_line - 1) / 2.0
     return CubeWorkload(f"raster_cube_{n // 1000}k_x{nchan}", lon, lat, vals,
                         Target(side, 0.01, round(ra_c, 6), round(dec_c, 6), side / 2.0), nchan)
+
+
+@dataclasses.dataclass
+class BatchSpec:
+    """One job of a batched workload (coordinates on the host, values made
+    on the device by batch_values)."""
+    name: str
+    lon: np.ndarray
+    lat: np.ndarray
+    target: Target
+    nchan: int
+    dtype: str
+    sigma: float
+    seed: int
+
+    @property
+    def n(self) -> int:
+        return int(self.lon.size)
+
+    @property
+    def radius(self) -> float:
+        return 3.0 * self.sigma
+
+    @property
+    def hpx_res(self) -> float:
+        return self.sigma / 2.0
+
+
+def batch_field(n: int, side: int, dec_c: float, ra_c: float = 180.0, cdelt: float = 0.01, seed: int = 0):
+    """Raster over a square field of side * cdelt degrees on the sky centred
+    on (ra_c, dec_c): round(sqrt(n)) constant-Dec lines, samples evenly
+    spaced in RA (spacing scaled by 1/cos(dec_c) so the field is square on
+    the sky) with a uniform jitter of +-1/6.4 of the spacing.  The centre is
+    moved towards the equator where the field would pass 89 deg."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    field = side * cdelt
+    half = field / 2.0
+    dec_c = float(np.clip(dec_c, -89.0 + half, 89.0 - half))
+    lines = max(1, int(round(math.sqrt(n))))
+    per_line = max(1, int(round(n / lines)))
+    ddec = field / lines
+    dra = field / per_line / math.cos(math.radians(dec_c))
+    k = np.arange(lines, dtype=np.float64)
+    j = np.arange(per_line, dtype=np.float64)
+    lat = np.repeat(dec_c - half + ddec * (k + 0.5), per_line)
+    lon = (np.tile(ra_c - dra * per_line / 2.0 + dra * (j + 0.5), lines)
+           + rng.uniform(-dra / 6.4, dra / 6.4, size=lines * per_line))
+    return lon, lat, Target(side, cdelt, ra_c, round(dec_c, 6), side / 2.0)
+
+
+def batch_specs(njobs: int = 64, seed: int = 3, sides=(128, 256, 512, 1024, 2048), chans=(1, 16, 64, 256, 512),
+                sigmas=(0.015, 0.02, 0.03), n_range=(5e4, 5e6), dec_range=(-60.0, 80.0)) -> list:
+    """Uneven jobs: N log-uniform in n_range, map side, channel count, value
+    type (bf16/f16) and sigma drawn uniformly from the lists, field centre
+    Dec uniform in dec_range at RA 180 deg (never near RA 0)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for i in range(njobs):
+        n = int(round(math.exp(rng.uniform(math.log(n_range[0]), math.log(n_range[1])))))
+        side = int(rng.choice(sides))
+        nchan = int(rng.choice(chans))
+        dtype = "bf16" if rng.integers(2) == 0 else "f16"
+        sigma = float(rng.choice(sigmas))
+        dec = float(rng.uniform(*dec_range))
+        lon, lat, tgt = batch_field(n, side, dec, seed=seed * 1000 + i)
+        out.append(BatchSpec(f"job{i}", lon, lat, tgt, nchan, dtype, sigma, seed * 1000 + i))
+    return out
+
+
+def batch_values(spec: BatchSpec, device: str = "cuda"):
+    """Channel-major values [nchan][n] of a job on the device: N(0, 0.5)
+    noise plus one Gaussian line (centre and width drawn per job) on a
+    spatial Gaussian of 0.3 x the field width around the field centre."""
+    import torch
+
+    gen = torch.Generator(device=device)
+    gen.manual_seed(spec.seed)
+    tdt = torch.bfloat16 if spec.dtype == "bf16" else torch.float16
+    n, c = spec.n, spec.nchan
+    rng = np.random.Generator(np.random.PCG64(spec.seed + 1))
+    mu = rng.uniform(0.2, 0.8) * c
+    wid = max(1.0, rng.uniform(0.02, 0.1) * c)
+    amp = rng.uniform(1.0, 10.0)
+    t = spec.target
+    scale = 0.3 * t.side * t.cdelt
+    dy = spec.lat - t.crval2
+    dx = (spec.lon - t.crval1) * math.cos(math.radians(t.crval2))
+    spat = torch.as_tensor(amp * np.exp(-(dx * dx + dy * dy) / (2 * scale * scale)), dtype=torch.float32,
+                           device=device)
+    ch = torch.arange(c, device=device, dtype=torch.float32)
+    prof = torch.exp(-((ch - mu) ** 2) / (2 * wid * wid))
+    out = torch.empty((c, n), dtype=tdt, device=device)
+    step = max(1, (1 << 26) // max(n, 1))
+    for c0 in range(0, c, step):
+        c1 = min(c, c0 + step)
+        blk = 0.5 * torch.randn((c1 - c0, n), generator=gen, device=device) + prof[c0:c1, None] * spat[None, :]
+        out[c0:c1] = blk.to(tdt)
+    return out

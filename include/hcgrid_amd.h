@@ -206,6 +206,28 @@ int hcg_grid_mfma(const hcg_healpix *hp, const hcg_kernel *kern, uint32_t side, 
                   const double *ring_s, const hcg_ring_meta *meta, const double *xwcs,
                   const double *ywcs, float *cube, float *wsum, int normalize, void *stream);
 
+/* Launch shape of the multi-channel kernel.  A workgroup owns 64 cells,
+ * a (64 / tile_cols) x tile_cols block of the map (tile_cols 8, 16, 32 or
+ * 64), with `waves` waves (1, 2, 4 or 8; 64 channels each; more channels
+ * take more workgroups, each recomputing the block's weights) and stages
+ * `chunk` samples at a time (128, 256 or 512).  0 in a field = the default
+ * (tile_cols 16; the fewest waves that cover the channels, at most 8; chunk
+ * by waves).  No counterpart in the reference: the shapes an autotuner
+ * picks per workload. */
+typedef struct hcg_mfma_shape {
+    uint32_t tile_cols, waves, chunk, reserved;
+} hcg_mfma_shape;
+
+/* hcg_grid_mfma with an explicit launch shape (NULL = defaults).  Results
+ * do not depend on the shape beyond FP32 summation order. */
+int hcg_grid_mfma_shaped(const hcg_healpix *hp, const hcg_kernel *kern, uint32_t side, uint32_t n,
+                         uint32_t nchan, int dtype, const hcg_sample *records, const uint32_t *keys_sorted,
+                         const double *lon_sorted, const double *lat_sorted, const double *weights_sorted,
+                         const void *values_sorted, const uint32_t *lut, const double *ring_z,
+                         const double *ring_s, const hcg_ring_meta *meta, const double *xwcs,
+                         const double *ywcs, float *cube, float *wsum, int normalize,
+                         const hcg_mfma_shape *shape, void *stream);
+
 /* Output normalisation of write_output_map (HCGrid/gmap.cpp:305-312):
  * map[c] = datacube[c] / weightscube[c] in FP64, NaN where the weight is 0.
  * map may alias datacube. */
@@ -223,6 +245,99 @@ int hcg_eval_exp_neg(uint32_t n, const double *x, double *out, void *stream);
 int hcg_wcs_sin_p2s(uint32_t naxis1, uint32_t naxis2, double crpix1, double crpix2,
                     double cdelt1, double cdelt2, double crval1, double crval2,
                     double lonpole, double *xwcs, double *ywcs);
+
+/* ---- resident geometries and batched multi-channel gridding ----
+ * No counterpart in the reference, which grids one map per run
+ * (HCGrid/HCGrid.cpp:38-197) and one channel per kernel pass
+ * (HCGrid+/gridding-omp.cu:552-678).  A geometry holds one map's sorted
+ * samples, ring LUT and target coordinates in HBM, so repeated requests on
+ * the same coordinates skip the pre-sort; a batch grids many (geometry,
+ * channel values, cube) jobs of uneven sizes in ONE persistent launch. */
+
+typedef struct hcg_geometry hcg_geometry;   /* opaque, device-resident */
+
+/* Device bytes hcg_geometry_create allocates for n samples and side x side
+ * cells at hp's nside. */
+size_t hcg_geometry_bytes(uint32_t n, uint32_t side, const hcg_healpix *hp);
+
+/* Allocates a geometry (copies hp and kern).  Build it before use. */
+int hcg_geometry_create(const hcg_healpix *hp, const hcg_kernel *kern, uint32_t n, uint32_t side,
+                        hcg_geometry **out);
+
+/* Enqueues keys + stable sort, sample records and ring LUT (hcg_presort,
+ * hcg_gather_samples, hcg_ring_lut) and copies the target coordinates
+ * (device, [side*side] degrees; skipped when they are the geometry's own
+ * arrays) into the geometry.  lon/lat: device, [n] degrees.  Kernels and
+ * device-to-device copies only: capturable into a graph. */
+int hcg_geometry_build(hcg_geometry *g, const double *lon_deg, const double *lat_deg, const double *xwcs,
+                       const double *ywcs, void *stream);
+
+/* Device arrays of a geometry (read-only views for the one-job entry points). */
+typedef struct hcg_geometry_view {
+    uint32_t n, side;
+    size_t bytes;
+    const uint32_t *keys_sorted, *perm;
+    const hcg_sample *records;
+    const double *lon_sorted, *lat_sorted;
+    const uint32_t *lut;
+    const double *ring_z, *ring_s;
+    const hcg_ring_meta *meta;
+    double *xwcs, *ywcs;
+} hcg_geometry_view;
+int hcg_geometry_arrays(const hcg_geometry *g, hcg_geometry_view *view);
+
+int hcg_geometry_destroy(hcg_geometry *g);
+
+/* One job of a batch. */
+typedef struct hcg_batch_job {
+    const hcg_geometry *geometry;
+    uint32_t nchan;
+    int dtype;                /* HCG_DTYPE_BF16 or HCG_DTYPE_F16 */
+    const void *values;       /* device, [nchan][n] channel-major */
+    float *cube;              /* device, [nchan][side*side] */
+    float *wsum;              /* device, [side*side], or NULL */
+    uint32_t tile_cols;       /* 8, 16, 32, 64, or 0 = 16 */
+    int normalize;            /* nonzero: cube = sum w v / sum w (NaN where no weight) */
+} hcg_batch_job;
+
+typedef struct hcg_batch hcg_batch;   /* opaque */
+
+/* Device bytes a job adds to its batch (its sorted channel values). */
+size_t hcg_batch_job_bytes(const hcg_geometry *g, uint32_t nchan);
+
+/* Plans a batch: orders the work items (tile x channel group of every job,
+ * longest items first), allocates the job table, work list, counters and
+ * the sorted channel buffers, and uploads the plan (synchronously).  waves
+ * per workgroup (1, 2, 4, 8; 0 = 4) and chunk (128, 256, 512; 0 = by waves)
+ * fix the persistent launch's shape.  The jobs' geometries, values and
+ * cubes must stay allocated while the batch is used. */
+int hcg_batch_create(uint32_t njobs, const hcg_batch_job *jobs, uint32_t waves, uint32_t chunk,
+                     hcg_batch **out);
+
+/* Enqueues the batch: counters cleared by a fill kernel, each job's channel
+ * gather, then one persistent launch over all work items.  Kernels only
+ * (capturable).  Results per job equal hcg_grid_mfma_shaped's with the same
+ * tile_cols, bit for bit. */
+int hcg_batch_enqueue(hcg_batch *b, void *stream);
+
+typedef struct hcg_batch_stats {
+    uint32_t njobs, nitems, blocks, waves, chunk, reserved;
+    size_t device_bytes;
+} hcg_batch_stats;
+int hcg_batch_info(const hcg_batch *b, hcg_batch_stats *stats);
+
+/* Per-job counts of non-finite channel values seen by the last run (host
+ * array of njobs; synchronises the stream). */
+int hcg_batch_nonfinite(const hcg_batch *b, uint32_t *counts, void *stream);
+
+int hcg_batch_destroy(hcg_batch *b);
+
+/* Graph capture of any sequence of the entry points above on a created
+ * stream (not the null stream), then replay.  No reference counterpart. */
+int hcg_graph_capture_begin(void *stream);
+int hcg_graph_capture_end(void *stream, void **graph_exec);
+int hcg_graph_launch(void *graph_exec, void *stream);
+int hcg_graph_destroy(void *graph_exec);
 
 #ifdef __cplusplus
 }

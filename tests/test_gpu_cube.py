@@ -28,7 +28,7 @@ def _values(wl, nchan, seed):
             rng.normal(0.0, 0.5, (nchan, wl.n))).astype(np.float32)
 
 
-def _run_case(n_lines, n_per, side, nchan, dtype, layout, seed, dec=30.0, tol=None):
+def _run_case(n_lines, n_per, side, nchan, dtype, layout, seed, dec=30.0, tol=None, shape=None):
     import torch
     wl = synth.raster_small(n_lines, n_per, side, dec_c=dec, seed=seed)
     t = wl.target
@@ -37,9 +37,9 @@ def _run_case(n_lines, n_per, side, nchan, dtype, layout, seed, dec=30.0, tol=No
     tdt = torch.bfloat16 if dtype == "bf16" else torch.float16
     Vt = torch.tensor(V).to(tdt).cuda()
     Vr = Vt.float().cpu().numpy().astype(np.float32)  # the rounded values both sides use
-    print(f"case: n={wl.n} side={side} nchan={nchan} dtype={dtype} layout={layout}", flush=True)
+    print(f"case: n={wl.n} side={side} nchan={nchan} dtype={dtype} layout={layout} shape={shape}", flush=True)
     dev = gridder.grid_cube(wl.lon, wl.lat, Vt if layout == 0 else Vt.t().contiguous(), xw, yw, side, wl.sigma,
-                            wl.radius, wl.hpx_res, dtype=dtype, layout=layout)
+                            wl.radius, wl.hpx_res, dtype=dtype, layout=layout, shape=shape)
     ref = oracle.pipeline(wl.lon, wl.lat, Vr[0], xw, yw, side, wl.sigma, wl.radius, wl.hpx_res)
     assert np.array_equal(dev["perm"], ref["perm"])
     out, ws = oracle.grid_compat_multi(ref["h"], ref["g"], ref["lon_s"], ref["lat_s"], Vr[:, ref["perm"]], None,
@@ -61,6 +61,13 @@ def _run_case(n_lines, n_per, side, nchan, dtype, layout, seed, dec=30.0, tol=No
 @pytest.mark.parametrize("nchan", [1, 48, 64, 130])
 def test_cube_bf16_channel_counts(gpu, nchan):
     _run_case(40, 200, 24, nchan, "bf16", 0, seed=nchan)
+
+
+@pytest.mark.parametrize("shape", [(8, 1, 128), (16, 2, 256), (32, 4, 512), (64, 8, 128), (16, 8, 512),
+                                   (8, 4, 256), (64, 1, 512)])
+def test_cube_launch_shapes(gpu, shape):
+    # every block shape, wave count and chunk; side 27 leaves partial blocks
+    _run_case(30, 160, 27, 100, "bf16", 0, seed=23, shape=shape)
 
 
 def test_cube_f16(gpu):
