@@ -43,9 +43,25 @@ def parse():
     ap.add_argument("--cube", dest="cube", action="store_true", default=True,
                     help="also measure the multi-channel path (2M samples x 1024 BF16 channels)")
     ap.add_argument("--no-cube", dest="cube", action="store_false")
+    ap.add_argument("--batch", dest="batch", action="store_true", default=True,
+                    help="also measure 64 uneven multi-channel jobs in one persistent launch (graph replay)")
+    ap.add_argument("--no-batch", dest="batch", action="store_false")
+    ap.add_argument("--batch-jobs", type=int, default=64)
     ap.add_argument("--calibrate-copy", dest="calibrate_copy", action="store_true",
                     help="first run one 1 GiB device-to-device copy (a reference for HBM byte counters)")
     return ap.parse_args()
+
+
+def measured_traffic(key: str):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE and
+    WRITE_SIZE, calibrated on a device copy of known size), kept in
+    profiles/traffic.json; None where no measurement is recorded."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            v = json.load(f).get(key)
+        return int(v) if v is not None else None
+    except (OSError, ValueError):
+        return None
 
 
 def grid_algorithmic_bytes(n: int, m: int) -> int:
@@ -224,10 +240,125 @@ def bench_cube(args, log):
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(f"k_grid_mfma/{wl.name}"),
             "algorithmic_bytes": abytes,
             "avg_ms": ms_grid,
         },
+    }
+
+
+def bench_batch(args, log):
+    """64 independent jobs (N log-uniform in [50k, 5M], map side in
+    {128..2048}, channels in {1, 16, 64, 256, 512}, BF16/FP16, sigma in
+    {0.015, 0.02, 0.03} deg, Dec in [-60, 80]) gridded by one graph: every
+    job's geometry build (keys, sort, records, LUT) and one persistent
+    launch over all jobs' tiles.  Block shapes per job class and the launch
+    shape are picked by the autotuner first (untimed).  Compared with the
+    single-channel path applied channel by channel to the same jobs."""
+    import torch
+
+    from hcgrid_amd import autotune, batch, gridder, synth
+
+    t0 = time.perf_counter()
+    specs = synth.batch_specs(njobs=args.batch_jobs)
+    jobs, keys = [], []
+    for sp in specs:
+        t = sp.target
+        xw, yw = gridder.wcs_sin(t.side, t.side, t.crpix, t.crpix, -t.cdelt, t.cdelt, t.crval1, t.crval2)
+        kern, hp = gridder.prepare_grid_kernel(1, [sp.sigma], sp.radius, sp.hpx_res)
+        d = lambda a: torch.as_tensor(a, dtype=torch.float64, device="cuda")  # noqa: E731
+        g = batch.Geometry(kern, hp, sp.n, t.side)
+        jobs.append(dict(geometry=g, values=synth.batch_values(sp), dtype=sp.dtype,
+                         cube=torch.empty((sp.nchan, t.side * t.side), dtype=torch.float32, device="cuda"),
+                         wsum=torch.empty(t.side * t.side, dtype=torch.float32, device="cuda"),
+                         lon=d(sp.lon), lat=d(sp.lat), xw=d(xw), yw=d(yw), kern=kern, hp=hp, spec=sp))
+        keys.append(autotune.job_class(sp.n, t.side, sp.nchan, sp.sigma, sp.dtype))
+    for j in jobs:
+        j["geometry"].build(j["lon"], j["lat"], j["xw"], j["yw"])
+    torch.cuda.synchronize()
+    updates = sum(sp.n * sp.nchan for sp in specs)
+    log(f"batch: {len(specs)} jobs, {sum(sp.n for sp in specs)} samples, {updates:.3e} sample-channel updates, "
+        f"set up in {time.perf_counter() - t0:.1f}s")
+
+    t1 = time.perf_counter()
+    tuner = autotune.Autotuner()
+    bjobs = [dict(geometry=j["geometry"], values=j["values"], cube=j["cube"], wsum=j["wsum"], dtype=j["dtype"])
+             for j in jobs]
+    bjobs = tuner.tune_jobs(bjobs, keys)
+    waves, chunk = tuner.launch_for(bjobs, "batch")
+    log(f"batch: autotuned {len(tuner.tiles)} job classes and the launch shape in {time.perf_counter() - t1:.1f}s "
+        f"(waves {waves}, chunk {chunk})")
+    b = batch.Batch(bjobs, waves=waves, chunk=chunk)
+    stream = torch.cuda.Stream()
+    torch.cuda.synchronize()
+
+    def seq(s):
+        for j in jobs:
+            j["geometry"].build(j["lon"], j["lat"], j["xw"], j["yw"], stream=s)
+        b.enqueue(stream=s)
+
+    graph = batch.Graph(seq, stream)
+    for _ in range(max(1, args.warmup)):
+        graph.launch()
+    stream.synchronize()
+    steps = max(1, min(args.steps, 5))
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    for e0, e1 in evs:
+        e0.record(stream)
+        graph.launch()
+        e1.record(stream)
+    stream.synchronize()
+    ms = sum(e0.elapsed_time(e1) for e0, e1 in evs) / steps
+    nonfinite = sum(b.nonfinite(stream))
+    st = b.stats()
+    log(f"batch: {ms:.2f} ms per graph replay ({st['nitems']} work items on {st['blocks']} workgroups); "
+        f"non-finite values {nonfinite}")
+
+    # the single-channel path on the same jobs, channel by channel
+    tot_ms = 0.0
+    for j in jobs:
+        sp = j["spec"]
+        pg = gridder.PlainGridder(sp.n, sp.target.side, j["kern"], j["hp"])
+        ch0 = j["values"][0].float().contiguous()
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+        pg.presort(j["lon"], j["lat"])
+        pg.ring_lut()
+        pg.gather(j["lon"], j["lat"], ch0)
+        pg.init_output()
+        pg.grid(j["xw"], j["yw"])
+        pg.normalize()
+        e[0].record()
+        pg.presort(j["lon"], j["lat"])
+        pg.ring_lut()
+        e[1].record()
+        for _ in range(2):
+            pg.gather(j["lon"], j["lat"], ch0)
+            pg.init_output()
+            pg.grid(j["xw"], j["yw"])
+            pg.normalize()
+        e[2].record()
+        torch.cuda.synchronize()
+        tot_ms += e[0].elapsed_time(e[1]) + sp.nchan * e[1].elapsed_time(e[2]) / 2
+        del pg
+    chan_rate = updates / (tot_ms * 1e-3)
+    rate = updates / (ms * 1e-3)
+    log(f"batch: channel-by-channel single-channel path on the same jobs: {tot_ms:.0f} ms; "
+        f"ratio {rate / chan_rate:.2f}x")
+    graph.close()
+    b.close()
+    return {
+        "workload": (f"{len(specs)} jobs: N log-uniform in [5e4, 5e6] (sum {sum(sp.n for sp in specs)}), side in "
+                     "{128..2048} at 0.01 deg, channels in {1,16,64,256,512} BF16/FP16, sigma in {0.015,0.02,0.03} "
+                     "deg, Dec in [-60, 80]; one graph = every job's geometry build + one persistent launch"),
+        "value": rate,
+        "unit": "sample·channel updates/s",
+        "ms_per_batch": ms,
+        "updates": updates,
+        "work_items": st["nitems"],
+        "launch": {"waves": waves, "chunk": chunk, "blocks": st["blocks"]},
+        "autotuned_classes": len(tuner.tiles),
+        "channel_by_channel": {"value": chan_rate, "ms_total": tot_ms},
+        "speedup_vs_channel_by_channel": rate / chan_rate,
     }
 
 
@@ -341,7 +472,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(f"k_grid_tiles/{wl.name}"),
             "algorithmic_bytes": abytes,
             "avg_ms": grid_avg,
             "note": "FP64-VALU-bound kernel: the HBM fraction is low by construction",
@@ -354,6 +485,10 @@ def main():
         del g, lon, lat, val, xwd, ywd
         torch.cuda.empty_cache()
         result["multichannel"] = bench_cube(args, log)
+        torch.cuda.empty_cache()
+    if args.batch:
+        result["batched"] = bench_batch(args, log)
+        torch.cuda.empty_cache()
     print(json.dumps(result), flush=True)
 
 

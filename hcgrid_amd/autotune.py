@@ -1,0 +1,119 @@
+"""Launch-shape autotuner for the multi-channel gridder.
+
+Two knobs, both without effect on results (the tests check bit equality):
+  * per job class, the block shape `tile_cols` (cells per workgroup form a
+    (64 / tile_cols) x tile_cols block of the map; squarer blocks re-read
+    fewer channel values, wider ones cross fewer rings);
+  * per batch, the persistent launch's `waves` per workgroup and `chunk`
+    (samples staged per step), which fix how many channels one work item
+    carries (waves x 64) and how many workgroups share a CU.
+
+A job class is (channel bucket, sigma, value type, log2 samples-per-cell
+bucket): jobs in one class have the same support in cells and the same
+balance of FP64 weight work against MFMA work, which is what the block
+shape trades.  The tuner times candidates on the device with events (one-job
+batches on a representative job for tile_cols, the whole batch for waves
+and chunk), keeps the fastest, and caches the choice in memory and
+optionally in a JSON file.
+"""
+from __future__ import annotations
+
+import json
+import math
+import os
+
+from . import batch as _batch
+
+TILE_COLS = (8, 16, 32, 64)
+LAUNCH_SHAPES = ((4, 256), (8, 512), (2, 128))
+
+
+def chan_bucket(nchan: int) -> int:
+    for b in (1, 16, 64, 256):
+        if nchan <= b:
+            return b
+    return 1024
+
+
+def job_class(n: int, side: int, nchan: int, sigma: float, dtype: str) -> str:
+    density = n / max(1, side * side)
+    dbin = int(math.floor(math.log2(density))) if density > 0 else -99
+    return f"c{chan_bucket(nchan)}_s{sigma:g}_{dtype}_d{dbin}"
+
+
+class Autotuner:
+    def __init__(self, cache_path: str | None = None, reps: int = 2):
+        self.cache_path = cache_path
+        self.reps = int(reps)
+        self.tiles: dict[str, int] = {}
+        self.launch: dict[str, list] = {}
+        self.log: list[str] = []
+        if cache_path and os.path.exists(cache_path):
+            with open(cache_path) as f:
+                d = json.load(f)
+            self.tiles = {k: int(v) for k, v in d.get("tiles", {}).items()}
+            self.launch = {k: list(v) for k, v in d.get("launch", {}).items()}
+
+    def save(self):
+        if self.cache_path:
+            with open(self.cache_path, "w") as f:
+                json.dump({"tiles": self.tiles, "launch": self.launch}, f, indent=1, sort_keys=True)
+
+    @staticmethod
+    def _time(fn, reps: int) -> float:
+        import torch
+        fn()  # warm
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    def tile_cols_for(self, job: dict, key: str) -> int:
+        """job: a Batch job dict whose geometry is built.  Times a one-job
+        batch per candidate block shape; returns (and caches) the fastest."""
+        if key in self.tiles:
+            return self.tiles[key]
+        best, best_ms = 16, float("inf")
+        times = {}
+        for tc in TILE_COLS:
+            j = dict(job, tile_cols=tc)
+            b = _batch.Batch([j])
+            ms = self._time(b.enqueue, self.reps)
+            b.close()
+            times[tc] = ms
+            if ms < best_ms:
+                best, best_ms = tc, ms
+        self.tiles[key] = best
+        self.log.append(f"{key}: tile_cols {best} ({', '.join(f'{k}:{v:.3f}ms' for k, v in times.items())})")
+        return best
+
+    def tune_jobs(self, jobs: list, keys: list[str]) -> list:
+        """Assigns tile_cols to every job (first job of each class is timed)."""
+        out = []
+        for j, k in zip(jobs, keys):
+            tc = self.tile_cols_for(j, k)
+            out.append(dict(j, tile_cols=tc))
+        return out
+
+    def launch_for(self, jobs: list, key: str) -> tuple[int, int]:
+        """(waves, chunk) of the persistent launch for this set of jobs."""
+        if key in self.launch:
+            w, c = self.launch[key]
+            return int(w), int(c)
+        best, best_ms = LAUNCH_SHAPES[0], float("inf")
+        times = {}
+        for w, c in LAUNCH_SHAPES:
+            b = _batch.Batch(jobs, waves=w, chunk=c)
+            ms = self._time(b.enqueue, 1)
+            b.close()
+            times[(w, c)] = ms
+            if ms < best_ms:
+                best, best_ms = (w, c), ms
+        self.launch[key] = list(best)
+        self.log.append(f"{key}: waves/chunk {best} ({', '.join(f'{k}:{v:.2f}ms' for k, v in times.items())})")
+        return best

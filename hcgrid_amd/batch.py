@@ -24,12 +24,23 @@ DTYPES = {"bf16": capi.DTYPE_BF16, "f16": capi.DTYPE_F16}
 class Geometry:
     """hcg_geometry: allocated at construction, filled by build()."""
 
-    def __init__(self, kern, hp, n: int, side: int):
+    def __init__(self, kern, hp, n: int, side: int, in_torch: bool = False):
+        """in_torch: place the geometry in a torch allocation (the caching
+        allocator), so creating and dropping it makes no hipMalloc/hipFree."""
         capi.load()
         self.kern, self.hp = kern, hp
         self.n, self.side = int(n), int(side)
         self._h = C.c_void_p()
-        call("hcg_geometry_create", C.byref(hp), C.byref(kern), self.n, self.side, C.byref(self._h))
+        self._mem = None
+        if in_torch:
+            torch = _torch()
+            nb = Geometry.bytes_for(self.n, self.side, hp)
+            self._mem = torch.empty(nb + 256, dtype=torch.uint8, device="cuda")
+            base = (self._mem.data_ptr() + 255) // 256 * 256
+            call("hcg_geometry_create_in", C.byref(hp), C.byref(kern), self.n, self.side, C.c_void_p(base), nb,
+                 C.byref(self._h))
+        else:
+            call("hcg_geometry_create", C.byref(hp), C.byref(kern), self.n, self.side, C.byref(self._h))
         v = GeometryView()
         call("hcg_geometry_arrays", self._h, C.byref(v))
         self.view = v
@@ -76,6 +87,7 @@ class Geometry:
         if self._h:
             call("hcg_geometry_destroy", self._h)
             self._h = C.c_void_p()
+        self._mem = None
 
     def __del__(self):
         try:
@@ -90,7 +102,10 @@ class Batch:
     f16), cube: device float32 [nchan][side*side], wsum: float32 [side*side]
     or None.  The batch keeps references to everything it reads or writes."""
 
-    def __init__(self, jobs, waves: int = 0, chunk: int = 0):
+    def __init__(self, jobs, waves: int = 0, chunk: int = 0, stream=None):
+        """stream: plan the batch in a torch allocation made on `stream` and
+        upload the plan asynchronously there (no hipMalloc/hipFree, no
+        synchronisation); the batch must then be enqueued on that stream."""
         capi.load()
         self.jobs = list(jobs)
         arr = (BatchJob * max(1, len(self.jobs)))()
@@ -114,7 +129,19 @@ class Batch:
             arr[i].tile_cols = int(j.get("tile_cols", 0))
             arr[i].normalize = 1 if j.get("normalize", True) else 0
         self._h = C.c_void_p()
-        call("hcg_batch_create", len(self.jobs), arr, int(waves), int(chunk), C.byref(self._h))
+        self._mem = None
+        if stream is None:
+            call("hcg_batch_create", len(self.jobs), arr, int(waves), int(chunk), C.byref(self._h))
+        else:
+            torch = _torch()
+            nb = int(capi.load().hcg_batch_bytes(len(self.jobs), arr, int(waves), int(chunk)))
+            if nb == 0:
+                raise capi.HcgError(capi.HCG_ERR_INVALID, "Batch", "invalid jobs")
+            with torch.cuda.stream(stream):
+                self._mem = torch.empty(nb + 256, dtype=torch.uint8, device="cuda")
+            base = (self._mem.data_ptr() + 255) // 256 * 256
+            call("hcg_batch_create_in", len(self.jobs), arr, int(waves), int(chunk), C.c_void_p(base), nb,
+                 _stream_handle(stream), C.byref(self._h))
 
     @property
     def handle(self) -> C.c_void_p:
@@ -141,6 +168,7 @@ class Batch:
         if self._h:
             call("hcg_batch_destroy", self._h)
             self._h = C.c_void_p()
+        self._mem = None
 
     def __del__(self):
         try:

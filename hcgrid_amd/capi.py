@@ -128,11 +128,15 @@ PROTOTYPES = {
     "hcg_wcs_sin_p2s": (_int, [_u32, _u32, _dbl, _dbl, _dbl, _dbl, _dbl, _dbl, _dbl, _vp, _vp]),
     "hcg_geometry_bytes": (_sz, [_u32, _u32, C.POINTER(Healpix)]),
     "hcg_geometry_create": (_int, [C.POINTER(Healpix), C.POINTER(Kernel), _u32, _u32, C.POINTER(_vp)]),
+    "hcg_geometry_create_in": (_int, [C.POINTER(Healpix), C.POINTER(Kernel), _u32, _u32, _vp, _sz,
+                                      C.POINTER(_vp)]),
     "hcg_geometry_build": (_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "hcg_geometry_arrays": (_int, [_vp, C.POINTER(GeometryView)]),
     "hcg_geometry_destroy": (_int, [_vp]),
     "hcg_batch_job_bytes": (_sz, [_vp, _u32]),
     "hcg_batch_create": (_int, [_u32, C.POINTER(BatchJob), _u32, _u32, C.POINTER(_vp)]),
+    "hcg_batch_bytes": (_sz, [_u32, C.POINTER(BatchJob), _u32, _u32]),
+    "hcg_batch_create_in": (_int, [_u32, C.POINTER(BatchJob), _u32, _u32, _vp, _sz, _vp, C.POINTER(_vp)]),
     "hcg_batch_enqueue": (_int, [_vp, _vp]),
     "hcg_batch_info": (_int, [_vp, C.POINTER(BatchStats)]),
     "hcg_batch_nonfinite": (_int, [_vp, C.POINTER(_u32), _vp]),

@@ -24,6 +24,7 @@ using namespace hcg;
 
 struct hcg_geometry {
   int device;
+  bool owns;             // base was allocated here (freed by destroy)
   hcg_healpix hp;
   hcg_kernel kern;
   uint32_t n, side;
@@ -46,6 +47,7 @@ struct hcg_geometry {
 
 struct hcg_batch {
   int device;
+  bool owns;             // base was allocated here (freed by destroy)
   uint32_t njobs, nitems, waves, chunk, blocks;
   size_t bytes;
   void* base;
@@ -54,6 +56,8 @@ struct hcg_batch {
   uint32_t* d_counters;  // [0] queue head, [1 + j] non-finite values of job j
   std::vector<hcg_batch_job> jobs;
   std::vector<void*> values_sorted;
+  std::vector<MfmaJob> host_jobs;     // upload sources, alive until destroy
+  std::vector<MfmaItem> host_items;
 };
 
 namespace {
@@ -120,8 +124,10 @@ extern "C" size_t hcg_geometry_bytes(uint32_t n, uint32_t side, const hcg_healpi
   return g.bytes;
 }
 
-extern "C" int hcg_geometry_create(const hcg_healpix* hp, const hcg_kernel* kern, uint32_t n, uint32_t side,
-                                   hcg_geometry** out) {
+namespace {
+
+int geometry_init(const hcg_healpix* hp, const hcg_kernel* kern, uint32_t n, uint32_t side, void* mem,
+                  size_t mem_bytes, hcg_geometry** out) {
   HCG_REQUIRE(hp && kern && out, "hcg_geometry_create: NULL argument");
   *out = nullptr;
   if (hp->nside == 0 || hp->nside > HCG_MAX_NSIDE || (hp->nside & (hp->nside - 1)))
@@ -137,7 +143,17 @@ extern "C" int hcg_geometry_create(const hcg_healpix* hp, const hcg_kernel* kern
   g->side = side;
   geometry_layout(g, nullptr);
   hipError_t e = hipGetDevice(&g->device);
-  if (e == hipSuccess) e = hipMalloc(&g->base, g->bytes);
+  if (e == hipSuccess && mem) {
+    if (mem_bytes < g->bytes || (reinterpret_cast<uintptr_t>(mem) & (kAlign - 1))) {
+      delete g;
+      return set_error(HCG_ERR_WORKSPACE, "hcg_geometry_create_in: memory smaller than hcg_geometry_bytes or not "
+                                          "256-byte aligned");
+    }
+    g->base = mem;
+  } else if (e == hipSuccess) {
+    e = hipMalloc(&g->base, g->bytes);
+    g->owns = true;
+  }
   if (e != hipSuccess) {
     delete g;
     (void)hipGetLastError();
@@ -148,9 +164,22 @@ extern "C" int hcg_geometry_create(const hcg_healpix* hp, const hcg_kernel* kern
   return HCG_OK;
 }
 
+}  // namespace
+
+extern "C" int hcg_geometry_create(const hcg_healpix* hp, const hcg_kernel* kern, uint32_t n, uint32_t side,
+                                   hcg_geometry** out) {
+  return geometry_init(hp, kern, n, side, nullptr, 0, out);
+}
+
+extern "C" int hcg_geometry_create_in(const hcg_healpix* hp, const hcg_kernel* kern, uint32_t n, uint32_t side,
+                                      void* device_mem, size_t bytes, hcg_geometry** out) {
+  HCG_REQUIRE(device_mem, "hcg_geometry_create_in: NULL memory");
+  return geometry_init(hp, kern, n, side, device_mem, bytes, out);
+}
+
 extern "C" int hcg_geometry_destroy(hcg_geometry* g) {
   if (!g) return HCG_OK;
-  if (g->base) HCG_CHECK_HIP(hipFree(g->base));
+  if (g->owns && g->base) HCG_CHECK_HIP(hipFree(g->base));
   delete g;
   return HCG_OK;
 }
@@ -197,10 +226,18 @@ extern "C" size_t hcg_batch_job_bytes(const hcg_geometry* g, uint32_t nchan) {
   return align_up((size_t)nchan * hcg_channels_padded_len(g->n) * 2);
 }
 
-extern "C" int hcg_batch_create(uint32_t njobs, const hcg_batch_job* jobs, uint32_t waves, uint32_t chunk,
-                                hcg_batch** out) {
-  HCG_REQUIRE(out, "hcg_batch_create: NULL out");
-  *out = nullptr;
+namespace {
+
+// A batch's plan: work items and the layout of its one device allocation.
+struct BatchPlan {
+  uint32_t waves = 0, chunk = 0;
+  std::vector<MfmaItem> items;
+  std::vector<uint32_t> tiles, tcols;
+  std::vector<size_t> voff;
+  size_t bytes = 0;
+};
+
+int plan_batch(uint32_t njobs, const hcg_batch_job* jobs, uint32_t waves, uint32_t chunk, BatchPlan& p) {
   HCG_REQUIRE(njobs == 0 || jobs, "hcg_batch_create: NULL jobs");
   if (waves == 0) waves = 4;
   if (chunk == 0) chunk = waves >= 8 ? 512 : (waves >= 4 ? 256 : 128);
@@ -216,70 +253,74 @@ extern "C" int hcg_batch_create(uint32_t njobs, const hcg_batch_job* jobs, uint3
                 "hcg_batch_create: unknown kernel type");
     HCG_REQUIRE(b.nchan == 0 || (b.values && b.cube), "hcg_batch_create: NULL values or cube");
   }
-  hcg_batch* bt = new (std::nothrow) hcg_batch{};
-  if (!bt) return set_error(HCG_ERR_NOMEM, "hcg_batch_create: host allocation failed");
-  bt->njobs = njobs;
-  bt->waves = waves;
-  bt->chunk = chunk;
-  bt->jobs.assign(jobs, jobs + njobs);
-
+  p.waves = waves;
+  p.chunk = chunk;
   // work items: jobs longest item first, each job's tiles in order, a
   // tile's channel groups adjacent (they share the tile's samples in L2)
-  std::vector<uint32_t> order(njobs);
+  std::vector<uint32_t> order(njobs), groups(njobs);
   std::vector<double> cost(njobs);
-  std::vector<uint32_t> tiles(njobs), groups(njobs), tcols(njobs);
+  p.tiles.assign(njobs, 0);
+  p.tcols.assign(njobs, 0);
   uint64_t nitems = 0;
   for (uint32_t j = 0; j < njobs; ++j) {
     const hcg_batch_job& b = jobs[j];
     const uint32_t tc = b.tile_cols ? b.tile_cols : kMcDefaultTileCols;
     const uint32_t side = b.geometry->side;
-    tcols[j] = tc;
-    tiles[j] = (b.nchan && side) ? ceil_div_u32(side, tc) * ceil_div_u32(side, 64 / tc) : 0;
+    p.tcols[j] = tc;
+    p.tiles[j] = (b.nchan && side) ? ceil_div_u32(side, tc) * ceil_div_u32(side, 64 / tc) : 0;
     groups[j] = ceil_div_u32(ceil_div_u32(b.nchan, kMcChanPerWave), waves);
     cost[j] = item_cost(b.geometry, tc, std::min<uint32_t>(b.nchan, waves * kMcChanPerWave));
     order[j] = j;
-    nitems += (uint64_t)tiles[j] * groups[j];
+    nitems += (uint64_t)p.tiles[j] * groups[j];
   }
-  if (nitems >= 0xFFFFFFFFull) {
-    delete bt;
-    return set_error(HCG_ERR_INVALID, "hcg_batch_create: too many work items");
-  }
+  if (nitems >= 0xFFFFFFFFull) return set_error(HCG_ERR_INVALID, "hcg_batch_create: too many work items");
   std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return cost[a] > cost[b]; });
-  std::vector<MfmaItem> items;
-  items.reserve(nitems);
+  p.items.clear();
+  p.items.reserve(nitems);
   for (uint32_t j : order)
-    for (uint32_t t = 0; t < tiles[j]; ++t)
-      for (uint32_t g = 0; g < groups[j]; ++g) items.push_back(MfmaItem{j, t, g, 0});
-  bt->nitems = (uint32_t)items.size();
-
+    for (uint32_t t = 0; t < p.tiles[j]; ++t)
+      for (uint32_t g = 0; g < groups[j]; ++g) p.items.push_back(MfmaItem{j, t, g, 0});
   // one allocation: job table, items, counters, sorted channel values
   Carver c{nullptr};
   c.take<MfmaJob>(njobs ? njobs : 1);
-  c.take<MfmaItem>(items.empty() ? 1 : items.size());
+  c.take<MfmaItem>(p.items.empty() ? 1 : p.items.size());
   c.take<uint32_t>(1 + njobs);
-  std::vector<size_t> voff(njobs);
+  p.voff.assign(njobs, 0);
   for (uint32_t j = 0; j < njobs; ++j) {
-    voff[j] = c.off;
+    p.voff[j] = c.off;
     c.off += hcg_batch_job_bytes(jobs[j].geometry, jobs[j].nchan);
   }
-  bt->bytes = c.off;
-  hipError_t e = hipGetDevice(&bt->device);
-  if (e == hipSuccess) e = hipMalloc(&bt->base, bt->bytes);
-  if (e != hipSuccess) {
-    delete bt;
-    (void)hipGetLastError();
-    return set_error(HCG_ERR_NOMEM, std::string("hcg_batch_create: hipMalloc: ") + hipGetErrorString(e));
-  }
+  p.bytes = c.off;
+  return HCG_OK;
+}
+
+// Lays the batch out in `mem` and uploads the job table and work list:
+// asynchronously on `stream` when upload_async, else synchronously.
+int batch_init(uint32_t njobs, const hcg_batch_job* jobs, const BatchPlan& p, void* mem, bool owns,
+               bool upload_async, hipStream_t stream, hcg_batch** out) {
+  hcg_batch* bt = new (std::nothrow) hcg_batch{};
+  if (!bt) return set_error(HCG_ERR_NOMEM, "hcg_batch_create: host allocation failed");
+  bt->njobs = njobs;
+  bt->waves = p.waves;
+  bt->chunk = p.chunk;
+  bt->nitems = (uint32_t)p.items.size();
+  bt->bytes = p.bytes;
+  bt->base = mem;
+  bt->owns = owns;
+  bt->jobs.assign(jobs, jobs + njobs);
+  (void)hipGetDevice(&bt->device);
   Carver d{static_cast<char*>(bt->base)};
   bt->d_jobs = d.take<MfmaJob>(njobs ? njobs : 1);
-  bt->d_items = d.take<MfmaItem>(items.empty() ? 1 : items.size());
+  bt->d_items = d.take<MfmaItem>(p.items.empty() ? 1 : p.items.size());
   bt->d_counters = d.take<uint32_t>(1 + njobs);
   bt->values_sorted.resize(njobs);
-  std::vector<MfmaJob> hj(njobs);
+  std::vector<MfmaJob>& hj = bt->host_jobs;
+  hj.resize(njobs);
+  bt->host_items = p.items;
   for (uint32_t j = 0; j < njobs; ++j) {
     const hcg_batch_job& b = jobs[j];
     const hcg_geometry* g = b.geometry;
-    bt->values_sorted[j] = static_cast<char*>(bt->base) + voff[j];
+    bt->values_sorted[j] = static_cast<char*>(bt->base) + p.voff[j];
     GridBuffers B{g->rec,  g->keys, g->lon_s, g->lat_s, nullptr, g->lut, g->ring_z,
                   g->ring_s, g->meta, g->xwcs,  g->ywcs,  nullptr, nullptr};
     hj[j] = MfmaJob{make_grid_params(&g->hp, &g->kern, g->side),
@@ -291,33 +332,70 @@ extern "C" int hcg_batch_create(uint32_t njobs, const hcg_batch_job* jobs, uint3
                     hcg_channels_padded_len(g->n),
                     b.nchan,
                     (uint32_t)b.dtype,
-                    tile_shift_of(tcols[j]),
-                    ceil_div_u32(g->side, tcols[j]),
-                    tiles[j],
+                    tile_shift_of(p.tcols[j]),
+                    ceil_div_u32(g->side, p.tcols[j]),
+                    p.tiles[j],
                     (uint32_t)(b.normalize ? 1 : 0)};
   }
-  e = hipSuccess;
-  if (njobs) e = hipMemcpy(bt->d_jobs, hj.data(), njobs * sizeof(MfmaJob), hipMemcpyHostToDevice);
-  if (e == hipSuccess && !items.empty())
-    e = hipMemcpy(bt->d_items, items.data(), items.size() * sizeof(MfmaItem), hipMemcpyHostToDevice);
-  // grid size of the persistent launch, fixed here so that enqueueing makes no queries
-  if (e == hipSuccess) {
-    uint32_t blocks = 0;
-    int rc = mfma_batch_blocks(chunk, waves, &blocks);
-    if (rc != HCG_OK) {
-      (void)hipFree(bt->base);
-      delete bt;
-      return rc;
-    }
-    bt->blocks = blocks;
+  hipError_t e = hipSuccess;
+  const size_t jb = njobs * sizeof(MfmaJob), ib = p.items.size() * sizeof(MfmaItem);
+  if (upload_async) {
+    // ordered on `stream`; the host sources live in the batch until destroy
+    if (jb) e = hipMemcpyAsync(bt->d_jobs, hj.data(), jb, hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess && ib)
+      e = hipMemcpyAsync(bt->d_items, bt->host_items.data(), ib, hipMemcpyHostToDevice, stream);
+  } else {
+    if (jb) e = hipMemcpy(bt->d_jobs, hj.data(), jb, hipMemcpyHostToDevice);
+    if (e == hipSuccess && ib) e = hipMemcpy(bt->d_items, p.items.data(), ib, hipMemcpyHostToDevice);
   }
-  if (e != hipSuccess) {
-    (void)hipFree(bt->base);
+  int rc = HCG_OK;
+  if (e != hipSuccess) rc = set_error(HCG_ERR_HIP, std::string("hcg_batch_create: ") + hipGetErrorString(e));
+  // grid size of the persistent launch, fixed here so that enqueueing makes no queries
+  if (rc == HCG_OK) rc = mfma_batch_blocks(p.chunk, p.waves, &bt->blocks);
+  if (rc != HCG_OK) {
+    if (owns) (void)hipFree(bt->base);
     delete bt;
-    return set_error(HCG_ERR_HIP, std::string("hcg_batch_create: ") + hipGetErrorString(e));
+    return rc;
   }
   *out = bt;
   return HCG_OK;
+}
+
+}  // namespace
+
+extern "C" size_t hcg_batch_bytes(uint32_t njobs, const hcg_batch_job* jobs, uint32_t waves, uint32_t chunk) {
+  BatchPlan p;
+  if (plan_batch(njobs, jobs, waves, chunk, p) != HCG_OK) return 0;
+  return p.bytes;
+}
+
+extern "C" int hcg_batch_create(uint32_t njobs, const hcg_batch_job* jobs, uint32_t waves, uint32_t chunk,
+                                hcg_batch** out) {
+  HCG_REQUIRE(out, "hcg_batch_create: NULL out");
+  *out = nullptr;
+  BatchPlan p;
+  int rc = plan_batch(njobs, jobs, waves, chunk, p);
+  if (rc != HCG_OK) return rc;
+  void* mem = nullptr;
+  hipError_t e = hipMalloc(&mem, p.bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return set_error(HCG_ERR_NOMEM, std::string("hcg_batch_create: hipMalloc: ") + hipGetErrorString(e));
+  }
+  return batch_init(njobs, jobs, p, mem, true, false, nullptr, out);
+}
+
+extern "C" int hcg_batch_create_in(uint32_t njobs, const hcg_batch_job* jobs, uint32_t waves, uint32_t chunk,
+                                   void* device_mem, size_t bytes, void* stream, hcg_batch** out) {
+  HCG_REQUIRE(out && device_mem, "hcg_batch_create_in: NULL argument");
+  *out = nullptr;
+  BatchPlan p;
+  int rc = plan_batch(njobs, jobs, waves, chunk, p);
+  if (rc != HCG_OK) return rc;
+  if (bytes < p.bytes || (reinterpret_cast<uintptr_t>(device_mem) & (kAlign - 1)))
+    return set_error(HCG_ERR_WORKSPACE, "hcg_batch_create_in: memory smaller than hcg_batch_bytes or not 256-byte "
+                                        "aligned");
+  return batch_init(njobs, jobs, p, device_mem, false, true, as_stream(stream), out);
 }
 
 extern "C" int hcg_batch_enqueue(hcg_batch* b, void* stream) {
@@ -360,7 +438,7 @@ extern "C" int hcg_batch_nonfinite(const hcg_batch* b, uint32_t* counts, void* s
 
 extern "C" int hcg_batch_destroy(hcg_batch* b) {
   if (!b) return HCG_OK;
-  if (b->base) HCG_CHECK_HIP(hipFree(b->base));
+  if (b->owns && b->base) HCG_CHECK_HIP(hipFree(b->base));
   delete b;
   return HCG_OK;
 }

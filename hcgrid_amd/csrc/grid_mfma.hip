@@ -415,22 +415,35 @@ int launch_mfma_batch(uint32_t chunk, uint32_t waves, uint32_t blocks, const Mfm
   return HCG_OK;
 }
 
-// Channel gather: sorted, channel-major, zero-padded to n_pad.
+// Channel gather: sorted, channel-major, zero-padded to n_pad.  A thread
+// owns one sorted slot and walks kGatherChans channels, so the permutation
+// is read once per kGatherChans channels rather than once per channel.
+constexpr uint32_t kGatherChans = 32;
+
 template <typename T>
-__global__ void __launch_bounds__(256) k_gather_channels_cm(uint32_t n, uint32_t n_pad, const uint32_t* __restrict__ perm,
+__global__ void __launch_bounds__(256) k_gather_channels_cm(uint32_t n, uint32_t n_pad, uint32_t nchan,
+                                                            const uint32_t* __restrict__ perm,
                                                             const T* __restrict__ in, T* __restrict__ out,
                                                             uint32_t* __restrict__ nonfinite) {
-  const uint32_t ch = blockIdx.y;
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n_pad) return;
-  T v = (T)0.0f;
+  const uint32_t ch0 = blockIdx.y * kGatherChans;
+  const uint32_t ch1 = min(nchan, ch0 + kGatherChans);
+  uint32_t j = 0;
   if (i < n) {
-    uint32_t j = perm[i];
+    j = perm[i];
     if (j >= n) j = i;
-    v = in[(size_t)ch * n + j];
-    if (nonfinite && !isfinite((float)v)) atomicAdd(nonfinite, 1u);
   }
-  out[(size_t)ch * n_pad + i] = v;
+  uint32_t bad = 0;
+  for (uint32_t ch = ch0; ch < ch1; ++ch) {
+    T v = (T)0.0f;
+    if (i < n) {
+      v = in[(size_t)ch * n + j];
+      bad += isfinite((float)v) ? 0u : 1u;
+    }
+    out[(size_t)ch * n_pad + i] = v;
+  }
+  if (nonfinite && bad) atomicAdd(nonfinite, bad);
 }
 
 // Sample-major input ([n][nchan]): 64 x 64 tiles transposed through LDS.
@@ -475,12 +488,13 @@ extern "C" int hcg_gather_channels(uint32_t n, uint32_t nchan, const uint32_t* p
   const uint32_t n_pad = hcg_channels_padded_len(n);
   hipStream_t s = as_stream(stream);
   if (layout == 0) {
-    const dim3 grid(ceil_div_u32(n_pad, 256), nchan);
+    const dim3 grid(ceil_div_u32(n_pad, 256), ceil_div_u32(nchan, kGatherChans));
     if (dtype == HCG_DTYPE_BF16)
-      k_gather_channels_cm<__bf16><<<grid, 256, 0, s>>>(n, n_pad, perm, static_cast<const __bf16*>(values),
+      k_gather_channels_cm<__bf16><<<grid, 256, 0, s>>>(n, n_pad, nchan, perm, static_cast<const __bf16*>(values),
                                                         static_cast<__bf16*>(values_sorted), nonfinite);
     else
-      k_gather_channels_cm<_Float16><<<grid, 256, 0, s>>>(n, n_pad, perm, static_cast<const _Float16*>(values),
+      k_gather_channels_cm<_Float16><<<grid, 256, 0, s>>>(n, n_pad, nchan, perm,
+                                                          static_cast<const _Float16*>(values),
                                                           static_cast<_Float16*>(values_sorted), nonfinite);
   } else {
     const dim3 grid(ceil_div_u32(n_pad, 64), ceil_div_u32(nchan, 64));

@@ -264,6 +264,12 @@ size_t hcg_geometry_bytes(uint32_t n, uint32_t side, const hcg_healpix *hp);
 int hcg_geometry_create(const hcg_healpix *hp, const hcg_kernel *kern, uint32_t n, uint32_t side,
                         hcg_geometry **out);
 
+/* The same in caller-provided device memory (>= hcg_geometry_bytes, 256-byte
+ * aligned), e.g. from a caching allocator: creating and destroying it then
+ * make no allocation call (hipFree synchronises the device). */
+int hcg_geometry_create_in(const hcg_healpix *hp, const hcg_kernel *kern, uint32_t n, uint32_t side,
+                           void *device_mem, size_t bytes, hcg_geometry **out);
+
 /* Enqueues keys + stable sort, sample records and ring LUT (hcg_presort,
  * hcg_gather_samples, hcg_ring_lut) and copies the target coordinates
  * (device, [side*side] degrees; skipped when they are the geometry's own
@@ -313,6 +319,16 @@ size_t hcg_batch_job_bytes(const hcg_geometry *g, uint32_t nchan);
  * cubes must stay allocated while the batch is used. */
 int hcg_batch_create(uint32_t njobs, const hcg_batch_job *jobs, uint32_t waves, uint32_t chunk,
                      hcg_batch **out);
+
+/* Device bytes hcg_batch_create_in needs for these jobs and launch shape
+ * (0 if the jobs are invalid). */
+size_t hcg_batch_bytes(uint32_t njobs, const hcg_batch_job *jobs, uint32_t waves, uint32_t chunk);
+
+/* hcg_batch_create in caller-provided device memory (>= hcg_batch_bytes,
+ * 256-byte aligned); the plan is uploaded asynchronously on `stream`, so
+ * the batch must be enqueued on that stream (or after it). */
+int hcg_batch_create_in(uint32_t njobs, const hcg_batch_job *jobs, uint32_t waves, uint32_t chunk,
+                        void *device_mem, size_t bytes, void *stream, hcg_batch **out);
 
 /* Enqueues the batch: counters cleared by a fill kernel, each job's channel
  * gather, then one persistent launch over all work items.  Kernels only
