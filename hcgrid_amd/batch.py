@@ -59,10 +59,20 @@ class Geometry:
     def nbytes(self) -> int:
         return int(self.view.bytes)
 
-    def build(self, lon, lat, xwcs, ywcs, stream=None):
-        """lon/lat: device float64 [n] degrees; xwcs/ywcs: device float64 [side*side]."""
-        call("hcg_geometry_build", self.handle, _ptr(lon), _ptr(lat), _ptr(xwcs), _ptr(ywcs),
-             _stream_handle(stream))
+    def build(self, lon, lat, xwcs=None, ywcs=None, stream=None):
+        """lon/lat: device float64 [n] degrees; xwcs/ywcs: device float64
+        [side*side], or None when the geometry's own target arrays
+        (view.xwcs/ywcs) have been filled already."""
+        xp = C.c_void_p(self.view.xwcs) if xwcs is None else _ptr(xwcs)
+        yp = C.c_void_p(self.view.ywcs) if ywcs is None else _ptr(ywcs)
+        call("hcg_geometry_build", self.handle, _ptr(lon), _ptr(lat), xp, yp, _stream_handle(stream))
+
+    def set_target_sin(self, target, stream=None):
+        """Fills the geometry's target coordinates on the device: a SIN map
+        of target.side^2 cells (hcg_wcs_sin_p2s_device)."""
+        t = target
+        call("hcg_wcs_sin_p2s_device", self.side, self.side, t.crpix, t.crpix, -t.cdelt, t.cdelt, t.crval1,
+             t.crval2, 180.0, C.c_void_p(self.view.xwcs), C.c_void_p(self.view.ywcs), _stream_handle(stream))
 
     def _read(self, ptr, count, dtype):
         torch = _torch()

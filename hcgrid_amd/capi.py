@@ -126,6 +126,7 @@ PROTOTYPES = {
     "hcg_normalize": (_int, [_u64, _vp, _vp, _vp, _vp]),
     "hcg_eval_exp_neg": (_int, [_u32, _vp, _vp, _vp]),
     "hcg_wcs_sin_p2s": (_int, [_u32, _u32, _dbl, _dbl, _dbl, _dbl, _dbl, _dbl, _dbl, _vp, _vp]),
+    "hcg_wcs_sin_p2s_device": (_int, [_u32, _u32, _dbl, _dbl, _dbl, _dbl, _dbl, _dbl, _dbl, _vp, _vp, _vp]),
     "hcg_geometry_bytes": (_sz, [_u32, _u32, C.POINTER(Healpix)]),
     "hcg_geometry_create": (_int, [C.POINTER(Healpix), C.POINTER(Kernel), _u32, _u32, C.POINTER(_vp)]),
     "hcg_geometry_create_in": (_int, [C.POINTER(Healpix), C.POINTER(Kernel), _u32, _u32, _vp, _sz,

@@ -175,6 +175,46 @@ extern "C" double hcg_host_pix2theta(const hcg_healpix* hp, uint64_t pix) { retu
 // PV2_2 = 0) and the native -> celestial rotation (Paper II eq. 2) with
 // (alpha_p, delta_p) = CRVAL and phi_p = LONPOLE.  Cell (x, y) <- pixel
 // (x+1, y+1) as set_WCS builds pixcrd (HCGrid/gmap.cpp:209-238).
+namespace {
+
+// One SIN-projection cell (Calabretta & Greisen 2002): intermediate world
+// coordinates (x, y) in radians -> (ra, dec) in degrees; NaN outside the
+// projection's boundary.
+__host__ __device__ inline void wcs_sin_cell(double x, double y, double a0, double sd0, double cd0, double php,
+                                             double& ra_deg, double& dec_deg) {
+  const double r2 = x * x + y * y;
+  if (r2 > 1.0) {
+    ra_deg = NAN;
+    dec_deg = NAN;
+    return;
+  }
+  // native latitude: cos(theta) = r, sin(theta) = sqrt(1 - r^2)
+  const double cth = sqrt(r2), sth = sqrt(1.0 - r2);
+  const double phi = atan2(x, -y);
+  const double dphi = phi - php;
+  const double sdp = sin(dphi), cdp = cos(dphi);
+  const double sdec = sth * sd0 + cth * cd0 * cdp;
+  const double dec = asin(sdec > 1.0 ? 1.0 : (sdec < -1.0 ? -1.0 : sdec));
+  double ra = (a0 + atan2(-cth * sdp, sth * cd0 - cth * sd0 * cdp)) * kRad2Deg;
+  ra = fmod(ra, 360.0);
+  if (ra < 0) ra += 360.0;
+  ra_deg = ra;
+  dec_deg = dec * kRad2Deg;
+}
+
+__global__ void __launch_bounds__(256) k_wcs_sin(uint32_t naxis1, uint32_t naxis2, double crpix1, double crpix2,
+                                                 double cdelt1, double cdelt2, double a0, double sd0, double cd0,
+                                                 double php, double* __restrict__ xwcs, double* __restrict__ ywcs) {
+  const uint64_t c = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (c >= (uint64_t)naxis1 * naxis2) return;
+  const uint32_t yy = (uint32_t)(c / naxis1), xx = (uint32_t)(c % naxis1);
+  const double y = cdelt2 * ((double)(yy + 1) - crpix2) * kDeg2Rad;
+  const double x = cdelt1 * ((double)(xx + 1) - crpix1) * kDeg2Rad;
+  wcs_sin_cell(x, y, a0, sd0, cd0, php, xwcs[c], ywcs[c]);
+}
+
+}  // namespace
+
 extern "C" int hcg_wcs_sin_p2s(uint32_t naxis1, uint32_t naxis2, double crpix1, double crpix2, double cdelt1,
                                double cdelt2, double crval1, double crval2, double lonpole, double* xwcs,
                                double* ywcs) {
@@ -185,26 +225,24 @@ extern "C" int hcg_wcs_sin_p2s(uint32_t naxis1, uint32_t naxis2, double crpix1, 
     const double y = cdelt2 * ((double)(yy + 1) - crpix2) * kDeg2Rad;
     for (uint32_t xx = 0; xx < naxis1; ++xx) {
       const double x = cdelt1 * ((double)(xx + 1) - crpix1) * kDeg2Rad;
-      const double r2 = x * x + y * y;
       const uint64_t c = (uint64_t)yy * naxis1 + xx;
-      if (r2 > 1.0) {  // outside the projection's boundary
-        xwcs[c] = NAN;
-        ywcs[c] = NAN;
-        continue;
-      }
-      // native latitude: cos(theta) = r, sin(theta) = sqrt(1 - r^2)
-      const double cth = sqrt(r2), sth = sqrt(1.0 - r2);
-      const double phi = atan2(x, -y);
-      const double dphi = phi - php;
-      const double sdp = sin(dphi), cdp = cos(dphi);
-      const double sdec = sth * sd0 + cth * cd0 * cdp;
-      const double dec = asin(sdec > 1.0 ? 1.0 : (sdec < -1.0 ? -1.0 : sdec));
-      double ra = (a0 + atan2(-cth * sdp, sth * cd0 - cth * sd0 * cdp)) * kRad2Deg;
-      ra = fmod(ra, 360.0);
-      if (ra < 0) ra += 360.0;
-      xwcs[c] = ra;
-      ywcs[c] = dec * kRad2Deg;
+      wcs_sin_cell(x, y, a0, sd0, cd0, php, xwcs[c], ywcs[c]);
     }
   }
+  return HCG_OK;
+}
+
+extern "C" int hcg_wcs_sin_p2s_device(uint32_t naxis1, uint32_t naxis2, double crpix1, double crpix2,
+                                      double cdelt1, double cdelt2, double crval1, double crval2, double lonpole,
+                                      double* xwcs, double* ywcs, void* stream) {
+  HCG_REQUIRE(xwcs && ywcs, "hcg_wcs_sin_p2s_device: NULL output");
+  const uint64_t m = (uint64_t)naxis1 * naxis2;
+  if (m == 0) return HCG_OK;
+  HCG_REQUIRE(m < (1ull << 40), "hcg_wcs_sin_p2s_device: too many cells");
+  const double a0 = crval1 * kDeg2Rad, d0 = crval2 * kDeg2Rad, php = lonpole * kDeg2Rad;
+  const double sd0 = sin(d0), cd0 = cos(d0);
+  k_wcs_sin<<<(uint32_t)((m + 255) / 256), 256, 0, as_stream(stream)>>>(naxis1, naxis2, crpix1, crpix2, cdelt1, cdelt2,
+                                                                      a0, sd0, cd0, php, xwcs, ywcs);
+  HCG_CHECK_LAUNCH("k_wcs_sin");
   return HCG_OK;
 }

@@ -99,8 +99,12 @@ struct MfmaLds {
   double x[CHUNK], y[CHUNK], z[CHUNK];
   uint32_t key[CHUNK];
   uint32_t mb[4][2];
-  uint32_t red[kMcMaxWaves][2];
   double wsum[kMcCols];
+  double p[kMcCols][3];          // cell unit vectors
+  uint32_t span[2];              // the block's ring span
+  uint32_t range[2][2];          // per ring parity: the union's sorted range
+  uint32_t lp[2][kMcCols];       // per ring parity: each cell's [lpix, rpix]
+  uint32_t rp[2][kMcCols];
 };
 
 // CHUNK = staged samples per chunk.  LDS: two weight tiles of 64 x (CHUNK+8)
@@ -126,7 +130,6 @@ __device__ __forceinline__ void mfma_block(const MfmaJob& J, uint32_t tile, uint
   double* s_z = L.z;
   uint32_t* s_key = L.key;
   auto& s_mb = L.mb;
-  auto& s_red = L.red;
   double* s_wsum = L.wsum;
   const GridParams& P = J.P;
   const GridBuffers& B = J.B;
@@ -155,16 +158,38 @@ __device__ __forceinline__ void mfma_block(const MfmaJob& J, uint32_t tile, uint
   const uint32_t col = x0 + (c & tc_mask);
   const bool live = col < P.side && row < P.side;
   const uint32_t cell = live ? row * P.side + col : 0;
-  const uint32_t anchor = live ? row * P.side + (col / P.factor) * P.factor : 0;
-  Window win = window_of(P, m.firstring, m.usedrings, B.xwcs[anchor], B.ywcs[anchor]);
-  if (!live || m.usedrings == 0) {
-    win.uring = 1;
-    win.dring = 0;
+
+  // Wave 0 holds the block's geometry, lane l = cell l: the cell's window
+  // and unit vector, the block's ring span, and per ring the cells'
+  // [lpix, rpix] and the union's sorted range, published through LDS.  The
+  // other waves only stage, weigh and contract.
+  Window win{};
+  if (wave == 0) {
+    const uint32_t rl = y0 + (lane >> tc_shift), cl = x0 + (lane & tc_mask);
+    const bool lv = cl < P.side && rl < P.side;
+    const uint32_t celll = lv ? rl * P.side + cl : 0;
+    const uint32_t anchor = lv ? rl * P.side + (cl / P.factor) * P.factor : 0;
+    win = window_of(P, m.firstring, m.usedrings, B.xwcs[anchor], B.ywcs[anchor]);
+    if (!lv || m.usedrings == 0) {
+      win.uring = 1;
+      win.dring = 0;
+    }
+    double sb, cb0, sn, cs;
+    sincos(B.ywcs[celll] * kDeg2Rad, &sb, &cb0);
+    sincos(B.xwcs[celll] * kDeg2Rad, &sn, &cs);
+    L.p[lane][0] = cb0 * cs;
+    L.p[lane][1] = cb0 * sn;
+    L.p[lane][2] = sb;
+    const bool has = win.uring <= win.dring;
+    const uint32_t ur = wave_min_u32(has ? win.uring : 0xFFFFFFFFu);
+    const uint32_t dr = wave_max_u32(has ? win.dring : 0u);
+    if (lane == 0) {
+      L.span[0] = ur;
+      L.span[1] = dr;
+    }
   }
-  double sb, cb0, sl, cl;
-  sincos(B.ywcs[cell] * kDeg2Rad, &sb, &cb0);
-  sincos(B.xwcs[cell] * kDeg2Rad, &sl, &cl);
-  const double px = cb0 * cl, py = cb0 * sl, pz = sb;
+  __syncthreads();
+  const double px = L.p[c][0], py = L.p[c][1], pz = L.p[c][2];
   double wsum = 0.0;
 
   // accumulators: 4 row blocks x kMcNB channel blocks of 16x16
@@ -184,50 +209,40 @@ __device__ __forceinline__ void mfma_block(const MfmaJob& J, uint32_t tile, uint
     bofs[nb] = (ch < nchan ? ch : nchan - 1);
   }
 
-  // ring span of the row segment
-  {
-    const bool has = win.uring <= win.dring;
-    const uint32_t ur = wave_min_u32(has ? win.uring : 0xFFFFFFFFu);
-    const uint32_t dr = wave_max_u32(has ? win.dring : 0u);
-    if (lane == 0) {
-      s_red[wave][0] = ur;
-      s_red[wave][1] = dr;
-    }
-  }
-  __syncthreads();
-  uint32_t r_begin = 0xFFFFFFFFu, r_end = 0;
-  for (uint32_t w = 0; w < nw; ++w) {
-    r_begin = min(r_begin, s_red[w][0]);
-    r_end = max(r_end, s_red[w][1]);
-  }
+  uint32_t r_begin = L.span[0];
+  const uint32_t r_end = L.span[1];
   if (r_begin == m.firstring) ++r_begin;  // the first used ring contributes nothing
 
   for (uint32_t r = r_begin; r <= r_end && r_begin <= r_end; ++r) {
     const uint32_t kr = r - m.firstring;
-    const uint32_t start = B.lut[kr], end = B.lut[kr + 1];
-    if (start == end) continue;  // uniform
-    uint64_t a64 = 1, b64 = 0;
-    const bool act =
-        r >= win.uring && r <= win.dring && ring_window(P, win, r, B.ring_z[kr], B.ring_s[kr], a64, b64) && a64 <= b64;
-    const uint32_t lp = act ? (uint32_t)a64 : 1u, rp = act ? (uint32_t)b64 : 0u;
-    __syncthreads();  // s_red reads of the previous round are done
-    {
-      const uint32_t wl = wave_min_u32(act ? lp : 0xFFFFFFFFu);
-      const uint32_t wr = wave_max_u32(act ? rp : 0u);
+    const uint32_t buf = r & 1u;  // ring-parity buffers: ring r+1 is written while ring r may still be read
+    if (wave == 0) {
+      const uint32_t start = B.lut[kr], end = B.lut[kr + 1];
+      uint32_t lo_r = 0, hi_r = 0;
+      if (start != end) {  // uniform
+        uint64_t a64 = 1, b64 = 0;
+        const bool act = r >= win.uring && r <= win.dring &&
+                         ring_window(P, win, r, B.ring_z[kr], B.ring_s[kr], a64, b64) && a64 <= b64;
+        const uint32_t lp = act ? (uint32_t)a64 : 1u, rp = act ? (uint32_t)b64 : 0u;
+        L.lp[buf][lane] = lp;  // (1, 0): no samples of this ring for the cell
+        L.rp[buf][lane] = rp;
+        const uint32_t LP = wave_min_u32(act ? lp : 0xFFFFFFFFu);
+        const uint32_t RP = wave_max_u32(act ? rp : 0u);
+        if (LP <= RP) {  // uniform
+          lo_r = wave_search<false>(B.keys, start, end, LP);
+          hi_r = wave_search<true>(B.keys, lo_r, end, RP);
+        }
+      }
       if (lane == 0) {
-        s_red[wave][0] = wl;
-        s_red[wave][1] = wr;
+        L.range[buf][0] = lo_r;
+        L.range[buf][1] = hi_r;
       }
     }
     __syncthreads();
-    uint32_t LP = 0xFFFFFFFFu, RP = 0;
-    for (uint32_t w = 0; w < nw; ++w) {
-      LP = min(LP, s_red[w][0]);
-      RP = max(RP, s_red[w][1]);
-    }
-    if (LP > RP) continue;  // uniform
-    const uint32_t lo_u = wave_search<false>(B.keys, start, end, LP);
-    const uint32_t hi_u = wave_search<true>(B.keys, lo_u, end, RP);
+    const uint32_t lo_u = L.range[buf][0], hi_u = L.range[buf][1];
+    if (lo_u >= hi_u) continue;  // uniform
+    const uint32_t lp = L.lp[buf][c], rp = L.rp[buf][c];
+    const bool act = lp <= rp;
     for (uint32_t c0 = lo_u & ~7u; c0 < hi_u; c0 += CHUNK) {
       const uint32_t cnt = min(CHUNK, hi_u - c0);
       const uint32_t nkb = (cnt + 31) / 32;

@@ -1,0 +1,342 @@
+"""Served gridding: continuous batching with HBM-footprint admission.
+
+Requests (one multi-channel gridding job each) arrive at any time.  The
+server keeps a small number of slots (streams).  Whenever a slot is free it
+forms a round from every pending request that fits the HBM budget (FIFO,
+no overtaking), enqueues the round on that slot's stream (geometry builds
+for new coordinates, the requests' channel data, one persistent batched
+launch) and goes back to admitting, so rounds on different slots overlap on
+the device while new requests keep arriving.  A round's memory returns to
+the budget when its completion event fires.
+
+Geometries (sorted samples, ring LUT, target coordinates) stay resident in
+an LRU cache keyed by the request's coordinate identity: a repeat request
+with new channel data skips the pre-sort.  Cached geometries count against
+the budget and are evicted, least recently used first, only when no round
+in flight uses them.
+
+The reference has no server (HCGrid/HCGrid.cpp:38-197 grids one map per
+process run); this is the request loop around the batched path.
+
+Inputs are synthesised on the device at admission (coordinates from the
+request's parameters and seed, values from its seed), so no host-to-device
+transfer is part of the measured latency; results stay in HBM.
+"""
+from __future__ import annotations
+
+import collections
+import dataclasses
+import queue
+import threading
+import time
+
+from . import autotune, batch, capi, gridder, synth
+
+
+@dataclasses.dataclass
+class Request:
+    rid: int
+    n: int
+    side: int
+    nchan: int
+    dtype: str
+    sigma: float
+    dec: float
+    geo_seed: int          # coordinates: same geo_seed (and shape) = same samples and target
+    val_seed: int          # channel data
+    arrival: float = 0.0
+    admitted: float = 0.0
+    done: float = 0.0
+    repeat: bool = False
+    geometry_hit: bool = False
+    footprint: int = 0
+    cube: object = None    # kept when the server keeps results
+    wsum: object = None
+
+    def __post_init__(self):
+        self.n = synth.field_size(self.n)  # the raster's actual sample count
+
+    @property
+    def geo_key(self) -> tuple:
+        return (self.n, self.side, self.sigma, round(self.dec, 9), self.geo_seed)
+
+    @property
+    def updates(self) -> int:
+        return self.n * self.nchan
+
+
+class _GeoEntry:
+    __slots__ = ("geometry", "bytes", "refs", "target", "last", "ready")
+
+    def __init__(self, geometry, nbytes, target):
+        self.geometry, self.bytes, self.target = geometry, nbytes, target
+        self.refs = 0
+        self.last = time.perf_counter()
+        self.ready = None
+
+
+class GriddingServer:
+    def __init__(self, budget_bytes: float = 200e9, slots: int = 2, waves: int = 4, chunk: int = 256,
+                 tuner: autotune.Autotuner | None = None, keep_results: bool = False, poll_s: float = 2e-4,
+                 max_round: int = 64):
+        import torch
+        self.torch = torch
+        capi.load()
+        self.budget = int(budget_bytes)
+        self.used = 0
+        self.waves, self.chunk = int(waves), int(chunk)
+        self.tuner = tuner
+        self.keep = keep_results
+        self.poll_s = poll_s
+        self.max_round = int(max_round)
+        self.inbox: queue.SimpleQueue = queue.SimpleQueue()
+        self.pending: collections.deque = collections.deque()
+        self.geoms: collections.OrderedDict = collections.OrderedDict()
+        self.streams = [torch.cuda.Stream() for _ in range(int(slots))]
+        self.rounds = [None] * int(slots)
+        self.finished: list[Request] = []
+        self.rejected: list[Request] = []
+        self.stats = collections.Counter()
+        self._kparams = {}
+
+    # -- client side ---------------------------------------------------------
+    def submit(self, req: Request):
+        if not req.arrival:
+            req.arrival = time.perf_counter()
+        self.inbox.put(req)
+
+    # -- footprint -------------------------------------------------------------
+    def _kern(self, sigma):
+        if sigma not in self._kparams:
+            self._kparams[sigma] = gridder.prepare_grid_kernel(1, [sigma], 3 * sigma, sigma / 2)
+        return self._kparams[sigma]
+
+    def _job_bytes(self, r: Request) -> int:
+        m = r.side * r.side
+        n_pad = int(capi.load().hcg_channels_padded_len(r.n))
+        values = r.nchan * r.n * 2
+        sorted_values = r.nchan * n_pad * 2 + 256
+        cube = r.nchan * m * 4 + m * 4
+        plan = 4096 + 16 * (-(-r.side // 8) * -(-r.side // 8)) * max(1, -(-r.nchan // (64 * self.waves)))
+        coords = 16 * r.n  # the request's lon/lat, alive until the round completes
+        return values + sorted_values + cube + plan + coords
+
+    def _geo_bytes(self, r: Request) -> int:
+        _, hp = self._kern(r.sigma)
+        return batch.Geometry.bytes_for(r.n, r.side, hp) + 256
+
+    def _evict(self, need: int) -> bool:
+        """Frees unreferenced cached geometries (LRU) until `need` more bytes fit."""
+        for key in list(self.geoms.keys()):
+            if self.used + need <= self.budget:
+                break
+            e = self.geoms[key]
+            if e.refs == 0:
+                self.used -= e.bytes
+                e.geometry.close()
+                del self.geoms[key]
+                self.stats["evictions"] += 1
+        return self.used + need <= self.budget
+
+    # -- rounds ----------------------------------------------------------------
+    def _admit(self) -> list:
+        """Pending requests (FIFO) that fit the budget now, up to max_round."""
+        take = []
+        reserved_geo = set()
+        while self.pending and len(take) < self.max_round:
+            r = self.pending[0]
+            need = self._job_bytes(r)
+            hit = r.geo_key in self.geoms or r.geo_key in reserved_geo
+            if not hit:
+                need += self._geo_bytes(r)
+            if self.used + need > self.budget and not self._evict(need):
+                in_flight = any(x is not None for x in self.rounds)
+                if not take and not in_flight and not any(e.refs for e in self.geoms.values()):
+                    # cannot fit even on an idle device
+                    self.pending.popleft()
+                    self.rejected.append(r)
+                    continue
+                break
+            self.pending.popleft()
+            self.used += need
+            r.footprint = need
+            r.geometry_hit = hit
+            if not hit:
+                reserved_geo.add(r.geo_key)
+            take.append(r)
+        return take
+
+    def _launch(self, slot: int, reqs: list):
+        torch = self.torch
+        s = self.streams[slot]
+        now = time.perf_counter()
+        jobs, keep = [], []
+        with torch.cuda.stream(s):
+            for r in reqs:
+                r.admitted = now
+                kern, hp = self._kern(r.sigma)
+                lon, lat, tgt = synth.device_field(r.n, r.side, r.dec, r.geo_seed)
+                e = self.geoms.get(r.geo_key)
+                if e is None:
+                    g = batch.Geometry(kern, hp, r.n, r.side, in_torch=True)
+                    g.set_target_sin(tgt, stream=s)
+                    g.build(lon, lat, stream=s)
+                    e = _GeoEntry(g, g.nbytes + 256, tgt)
+                    e.ready = torch.cuda.Event()
+                    e.ready.record(s)
+                    self.geoms[r.geo_key] = e
+                    self.stats["geometry_builds"] += 1
+                    r.footprint -= e.bytes  # the cached geometry stays charged until evicted
+                else:
+                    self.geoms.move_to_end(r.geo_key)
+                    s.wait_event(e.ready)  # built on another slot: wait for it on the device
+                    self.stats["geometry_hits"] += 1
+                e.refs += 1
+                e.last = now
+                vals = synth.device_values(r.nchan, r.dtype, lon, lat, tgt, r.val_seed)
+                keep.append((lon, lat))
+                cube = torch.empty((r.nchan, r.side * r.side), dtype=torch.float32, device="cuda")
+                wsum = torch.empty(r.side * r.side, dtype=torch.float32, device="cuda")
+                tc = 0
+                if self.tuner is not None:
+                    tc = self.tuner.tiles.get(autotune.job_class(r.n, r.side, r.nchan, r.sigma, r.dtype), 0)
+                jobs.append(dict(geometry=e.geometry, values=vals, cube=cube, wsum=wsum, dtype=r.dtype,
+                                 tile_cols=tc))
+                if self.keep:
+                    r.cube, r.wsum = cube, wsum
+            b = batch.Batch(jobs, waves=self.waves, chunk=self.chunk, stream=s)
+            b.enqueue(stream=s)
+            ev = torch.cuda.Event()
+            ev.record(s)
+        self.rounds[slot] = (reqs, b, jobs, keep, ev)
+        self.stats["rounds"] += 1
+        self.stats["round_jobs_max"] = max(self.stats["round_jobs_max"], len(reqs))
+
+    def _reap(self) -> int:
+        done = 0
+        for slot, rd in enumerate(self.rounds):
+            if rd is None:
+                continue
+            reqs, b, jobs, keep, ev = rd
+            if not ev.query():
+                continue
+            t = time.perf_counter()
+            for r in reqs:
+                r.done = t
+                self.used -= r.footprint
+                self.geoms[r.geo_key].refs -= 1
+                self.finished.append(r)
+            b.close()
+            self.rounds[slot] = None
+            done += len(reqs)
+        return done
+
+    # -- main loop -----------------------------------------------------------------
+    def serve(self, n_total: int, timeout_s: float = 3600.0):
+        """Runs until n_total requests have finished or been rejected."""
+        t_end = time.perf_counter() + timeout_s
+        while len(self.finished) + len(self.rejected) < n_total:
+            if time.perf_counter() > t_end:
+                raise TimeoutError(f"served {len(self.finished)} of {n_total} requests before the timeout")
+            busy = False
+            while True:
+                try:
+                    self.pending.append(self.inbox.get_nowait())
+                except queue.Empty:
+                    break
+            if self._reap():
+                busy = True
+            for slot in range(len(self.rounds)):
+                if self.rounds[slot] is None and self.pending:
+                    reqs = self._admit()
+                    if reqs:
+                        self._launch(slot, reqs)
+                        busy = True
+            if not busy:
+                time.sleep(self.poll_s)
+        self.torch.cuda.synchronize()
+        return self.finished
+
+    def close(self):
+        self.torch.cuda.synchronize()
+        for e in self.geoms.values():
+            e.geometry.close()
+        self.geoms.clear()
+        self.used = 0
+
+
+def request_stream(n_requests: int, rate: float, seed: int = 4, repeat_frac: float = 0.3, recent: int = 32,
+                   **dist) -> list:
+    """Poisson arrivals (rate per second) of jobs drawn like batch_specs;
+    a repeat_frac share reuses the coordinates and target of one of the last
+    `recent` distinct maps with new channel data.  Returns (offset_s, Request)."""
+    import math
+
+    import numpy as np
+
+    rng = np.random.Generator(np.random.PCG64(seed))
+    sides = dist.get("sides", (128, 256, 512, 1024, 2048))
+    chans = dist.get("chans", (1, 16, 64, 256, 512))
+    sigmas = dist.get("sigmas", (0.015, 0.02, 0.03))
+    n_lo, n_hi = dist.get("n_range", (5e4, 5e6))
+    dec_lo, dec_hi = dist.get("dec_range", (-60.0, 80.0))
+    t = 0.0
+    out, originals = [], []
+    for i in range(n_requests):
+        t += rng.exponential(1.0 / rate)
+        nchan = int(rng.choice(chans))
+        dtype = "bf16" if rng.integers(2) == 0 else "f16"
+        if originals and rng.uniform() < repeat_frac:
+            o = originals[-recent:][int(rng.integers(min(recent, len(originals))))]
+            r = Request(i, o.n, o.side, nchan, dtype, o.sigma, o.dec, o.geo_seed, seed * 100003 + i, repeat=True)
+        else:
+            n = int(round(math.exp(rng.uniform(math.log(n_lo), math.log(n_hi)))))
+            side = int(rng.choice(sides))
+            sigma = float(rng.choice(sigmas))
+            dec = float(rng.uniform(dec_lo, dec_hi))
+            r = Request(i, n, side, nchan, dtype, sigma, dec, seed * 1000003 + i, seed * 100003 + i)
+            originals.append(r)
+        out.append((t, r))
+    return out
+
+
+def run_load(server: GriddingServer, stream: list, time_scale: float = 1.0):
+    """Submits the requests at their arrival offsets from a client thread
+    while the server loop runs in this thread; returns the finished list."""
+    t0 = time.perf_counter()
+
+    def client():
+        for off, r in stream:
+            wait = t0 + off * time_scale - time.perf_counter()
+            if wait > 0:
+                time.sleep(wait)
+            r.arrival = t0 + off * time_scale
+            server.submit(r)
+
+    th = threading.Thread(target=client, daemon=True)
+    th.start()
+    done = server.serve(len(stream))
+    th.join()
+    return done
+
+
+def latency_summary(done: list) -> dict:
+    import numpy as np
+
+    lat = np.array([r.done - r.arrival for r in done])
+    qd = np.array([r.admitted - r.arrival for r in done])
+    t0 = min(r.arrival for r in done)
+    t1 = max(r.done for r in done)
+    upd = sum(r.updates for r in done)
+    return {
+        "jobs": len(done),
+        "p50_ms": float(np.percentile(lat, 50) * 1e3),
+        "p99_ms": float(np.percentile(lat, 99) * 1e3),
+        "mean_ms": float(lat.mean() * 1e3),
+        "queue_p50_ms": float(np.percentile(qd, 50) * 1e3),
+        "wall_s": t1 - t0,
+        "updates": upd,
+        "value": upd / (t1 - t0),
+        "repeats": int(sum(r.repeat for r in done)),
+        "geometry_hits": int(sum(r.geometry_hit for r in done)),
+    }

@@ -275,3 +275,61 @@ def batch_values(spec: BatchSpec, device: str = "cuda"):
         blk = 0.5 * torch.randn((c1 - c0, n), generator=gen, device=device) + prof[c0:c1, None] * spat[None, :]
         out[c0:c1] = blk.to(tdt)
     return out
+
+
+def field_size(n: int) -> int:
+    """Samples in a batch_field/device_field raster asked for n samples:
+    round(sqrt(n)) lines of round(n / lines) samples."""
+    lines = max(1, int(round(math.sqrt(n))))
+    return lines * max(1, int(round(n / lines)))
+
+
+def device_field(n: int, side: int, dec_c: float, seed: int, ra_c: float = 180.0, cdelt: float = 0.01,
+                 device: str = "cuda"):
+    """batch_field's raster made on the device (torch, on the current
+    stream; the jitter comes from torch's generator, so the samples differ
+    from batch_field's).  Returns (lon, lat, Target), float64 tensors."""
+    import torch
+
+    field = side * cdelt
+    half = field / 2.0
+    dec_c = float(np.clip(dec_c, -89.0 + half, 89.0 - half))
+    lines = max(1, int(round(math.sqrt(n))))
+    per_line = max(1, int(round(n / lines)))
+    ddec = field / lines
+    dra = field / per_line / math.cos(math.radians(dec_c))
+    gen = torch.Generator(device=device)
+    gen.manual_seed(int(seed))
+    k = torch.arange(lines, dtype=torch.float64, device=device)
+    j = torch.arange(per_line, dtype=torch.float64, device=device)
+    lat = (dec_c - half + ddec * (k + 0.5)).repeat_interleave(per_line)
+    jit = (torch.rand(lines * per_line, generator=gen, device=device, dtype=torch.float64) * 2 - 1) * (dra / 6.4)
+    lon = (ra_c - dra * per_line / 2.0 + dra * (j + 0.5)).repeat(lines) + jit
+    return lon, lat, Target(side, cdelt, ra_c, round(dec_c, 6), side / 2.0)
+
+
+def device_values(nchan: int, dtype: str, lon, lat, target: Target, seed: int, device: str = "cuda"):
+    """batch_values made entirely on the device from device coordinates."""
+    import torch
+
+    gen = torch.Generator(device=device)
+    gen.manual_seed(int(seed) + 7)
+    tdt = torch.bfloat16 if dtype == "bf16" else torch.float16
+    n = int(lon.numel())
+    rng = np.random.Generator(np.random.PCG64(int(seed) + 1))
+    mu = rng.uniform(0.2, 0.8) * nchan
+    wid = max(1.0, rng.uniform(0.02, 0.1) * nchan)
+    amp = rng.uniform(1.0, 10.0)
+    scale = 0.3 * target.side * target.cdelt
+    dy = lat - target.crval2
+    dx = (lon - target.crval1) * math.cos(math.radians(target.crval2))
+    spat = (amp * torch.exp(-(dx * dx + dy * dy) / (2 * scale * scale))).float()
+    ch = torch.arange(nchan, device=device, dtype=torch.float32)
+    prof = torch.exp(-((ch - mu) ** 2) / (2 * wid * wid))
+    out = torch.empty((nchan, n), dtype=tdt, device=device)
+    step = max(1, (1 << 26) // max(n, 1))
+    for c0 in range(0, nchan, step):
+        c1 = min(nchan, c0 + step)
+        blk = 0.5 * torch.randn((c1 - c0, n), generator=gen, device=device) + prof[c0:c1, None] * spat[None, :]
+        out[c0:c1] = blk.to(tdt)
+    return out

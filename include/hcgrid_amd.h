@@ -246,6 +246,14 @@ int hcg_wcs_sin_p2s(uint32_t naxis1, uint32_t naxis2, double crpix1, double crpi
                     double cdelt1, double cdelt2, double crval1, double crval2,
                     double lonpole, double *xwcs, double *ywcs);
 
+/* The same projection evaluated on the device (device outputs, enqueued on
+ * stream).  Same formula and operation order; the device sin/cos/atan2/asin
+ * are the HIP math library's, so results may differ from the host form in
+ * the last bit. */
+int hcg_wcs_sin_p2s_device(uint32_t naxis1, uint32_t naxis2, double crpix1, double crpix2,
+                           double cdelt1, double cdelt2, double crval1, double crval2,
+                           double lonpole, double *xwcs, double *ywcs, void *stream);
+
 /* ---- resident geometries and batched multi-channel gridding ----
  * No counterpart in the reference, which grids one map per run
  * (HCGrid/HCGrid.cpp:38-197) and one channel per kernel pass
