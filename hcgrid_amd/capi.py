@@ -120,6 +120,10 @@ PROTOTYPES = {
     "hcg_gather_channels": (_int, [_u32, _u32, _vp, _vp, _int, _int, _vp, _vp, _vp]),
     "hcg_grid_mfma": (_int, [C.POINTER(Healpix), C.POINTER(Kernel), _u32, _u32, _u32, _int, _vp, _vp, _vp, _vp,
                              _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _int, _vp]),
+    "hcg_grid_mfma_workspace_size": (_sz, [_u32, C.POINTER(MfmaShape)]),
+    "hcg_grid_mfma_ex": (_int, [C.POINTER(Healpix), C.POINTER(Kernel), _u32, _u32, _u32, _int, _vp, _vp, _vp,
+                                _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _int,
+                                C.POINTER(MfmaShape), _vp, _sz, _vp]),
     "hcg_grid_mfma_shaped": (_int, [C.POINTER(Healpix), C.POINTER(Kernel), _u32, _u32, _u32, _int, _vp, _vp, _vp,
                                     _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _int,
                                     C.POINTER(MfmaShape), _vp]),

@@ -233,7 +233,7 @@ struct BatchPlan {
   uint32_t waves = 0, chunk = 0;
   std::vector<MfmaItem> items;
   std::vector<uint32_t> tiles, tcols;
-  std::vector<size_t> voff;
+  std::vector<size_t> voff, poff;
   size_t bytes = 0;
 };
 
@@ -286,9 +286,12 @@ int plan_batch(uint32_t njobs, const hcg_batch_job* jobs, uint32_t waves, uint32
   c.take<MfmaItem>(p.items.empty() ? 1 : p.items.size());
   c.take<uint32_t>(1 + njobs);
   p.voff.assign(njobs, 0);
+  p.poff.assign(njobs, 0);
   for (uint32_t j = 0; j < njobs; ++j) {
     p.voff[j] = c.off;
     c.off += hcg_batch_job_bytes(jobs[j].geometry, jobs[j].nchan);
+    p.poff[j] = c.off;
+    c.off = align_up(c.off + (size_t)p.tiles[j] * kPlanStride * sizeof(uint32_t));
   }
   p.bytes = c.off;
   return HCG_OK;
@@ -335,7 +338,8 @@ int batch_init(uint32_t njobs, const hcg_batch_job* jobs, const BatchPlan& p, vo
                     tile_shift_of(p.tcols[j]),
                     ceil_div_u32(g->side, p.tcols[j]),
                     p.tiles[j],
-                    (uint32_t)(b.normalize ? 1 : 0)};
+                    (uint32_t)(b.normalize ? 1 : 0),
+                    reinterpret_cast<uint32_t*>(static_cast<char*>(bt->base) + p.poff[j])};
   }
   hipError_t e = hipSuccess;
   const size_t jb = njobs * sizeof(MfmaJob), ib = p.items.size() * sizeof(MfmaItem);
@@ -409,6 +413,8 @@ extern "C" int hcg_batch_enqueue(hcg_batch* b, void* stream) {
     if (jb.nchan == 0) continue;
     int rc = hcg_gather_channels(jb.geometry->n, jb.nchan, jb.geometry->perm, jb.values, jb.dtype, 0,
                                  b->values_sorted[j], b->d_counters + 1 + j, stream);
+    if (rc != HCG_OK) return rc;
+    rc = launch_mfma_plan(b->host_jobs[j], s);
     if (rc != HCG_OK) return rc;
   }
   if (b->nitems == 0) return HCG_OK;

@@ -73,11 +73,14 @@ __device__ inline float widen16(uint16_t h, bool bf) {
   return bf ? (float)__builtin_bit_cast(__bf16, h) : (float)__builtin_bit_cast(_Float16, h);
 }
 
-// One k-block of B for kMcNB channel blocks: 16 bytes per lane per block
-// from row ch of the sorted channel-major values (row length n_pad).
-__device__ inline void load_b_rows(s16x8_t* b, const uint16_t* p, const uint32_t* ch, uint32_t n_pad) {
+// One k-block of B for kMcNB channel blocks, 16 bytes per lane per block.
+// The sorted channel values are blocked by 32 samples: block g holds
+// [nchan][32] values, so p points at block g + the lane's sample offset and
+// chofs[nb] = 32 x the lane's channel; a wave's 64 channels of a block are
+// one contiguous 4 KB piece.
+__device__ inline void load_b_rows(s16x8_t* b, const uint16_t* p, const uint32_t* chofs) {
 #pragma unroll
-  for (int nb = 0; nb < kMcNB; ++nb) b[nb] = *reinterpret_cast<const s16x8_t*>(p + (size_t)ch[nb] * n_pad);
+  for (int nb = 0; nb < kMcNB; ++nb) b[nb] = *reinterpret_cast<const s16x8_t*>(p + chofs[nb]);
 }
 
 // FP64 weight of a selected pair (the single-channel kernel's evaluation).
@@ -91,6 +94,106 @@ __device__ inline double pair_weight_any(const GridParams& P, const GridBuffers&
                                           B.lat_s, 0.25 * c2, cell, i);
 }
 
+// Block geometry, lane l = cell l of a (64 >> tc_shift) x (1 << tc_shift)
+// block at (y0, x0): the cell's window (its anchor's, HCGrid/gridding.cu:306-323)
+// and unit vector q, and the block's ring span [ur, dr] (a wave reduction;
+// ur > dr when no cell has rings), first used ring excluded.
+__device__ inline Window block_cell_window(const GridParams& P, const GridBuffers& B, const hcg_ring_meta& m,
+                                           uint32_t y0, uint32_t x0, uint32_t tc_shift, uint32_t lane, double q[3],
+                                           uint32_t& ur, uint32_t& dr) {
+  const uint32_t tc_mask = (1u << tc_shift) - 1u;
+  const uint32_t rl = y0 + (lane >> tc_shift), cl = x0 + (lane & tc_mask);
+  const bool lv = cl < P.side && rl < P.side;
+  const uint32_t celll = lv ? rl * P.side + cl : 0;
+  const uint32_t anchor = lv ? rl * P.side + (cl / P.factor) * P.factor : 0;
+  Window win = window_of(P, m.firstring, m.usedrings, B.xwcs[anchor], B.ywcs[anchor]);
+  if (!lv || m.usedrings == 0) {
+    win.uring = 1;
+    win.dring = 0;
+  }
+  double sb, cb0, sn, cs;
+  sincos(B.ywcs[celll] * kDeg2Rad, &sb, &cb0);
+  sincos(B.xwcs[celll] * kDeg2Rad, &sn, &cs);
+  q[0] = cb0 * cs;
+  q[1] = cb0 * sn;
+  q[2] = sb;
+  const bool has = win.uring <= win.dring;
+  ur = wave_min_u32(has ? win.uring : 0xFFFFFFFFu);
+  dr = wave_max_u32(has ? win.dring : 0u);
+  if (ur == m.firstring) ++ur;  // the first used ring contributes nothing (start_int - 1 wraps)
+  return win;
+}
+
+// A cell's [lpix, rpix] in ring r (HCGrid/gridding.cu:334-360); (1, 0) when
+// the cell takes no samples from the ring.
+__device__ inline void ring_cell_window(const GridParams& P, const GridBuffers& B, const hcg_ring_meta& m,
+                                        const Window& win, uint32_t r, uint32_t& lp, uint32_t& rp) {
+  const uint32_t kr = r - m.firstring;
+  uint64_t a64 = 1, b64 = 0;
+  const bool act =
+      r >= win.uring && r <= win.dring && ring_window(P, win, r, B.ring_z[kr], B.ring_s[kr], a64, b64) && a64 <= b64;
+  lp = act ? (uint32_t)a64 : 1u;
+  rp = act ? (uint32_t)b64 : 0u;
+}
+
+// The sorted range of the union of the wave's windows within [start, end)
+// (a 64-way cooperative search of the keys); lo >= hi when empty.
+__device__ inline void ring_union_range(const GridBuffers& B, uint32_t lp, uint32_t rp, uint32_t start, uint32_t end,
+                                        uint32_t& lo, uint32_t& hi) {
+  const bool act = lp <= rp;
+  const uint32_t LP = wave_min_u32(act ? lp : 0xFFFFFFFFu);
+  const uint32_t RP = wave_max_u32(act ? rp : 0u);
+  lo = hi = 0;
+  if (LP <= RP) {  // uniform
+    lo = wave_search<false>(B.keys, start, end, LP);
+    hi = wave_search<true>(B.keys, lo, end, RP);
+  }
+}
+
+// The tile plan: one wave per tile, lane = cell.  Many small workgroups per
+// CU hide the latency of the per-ring searches that the gridding kernel,
+// with one or two large workgroups per CU, cannot.
+__global__ void __launch_bounds__(64) k_mfma_plan(MfmaJob J) {
+  const uint32_t tile = blockIdx.x;
+  const uint32_t lane = threadIdx.x;
+  const hcg_ring_meta m = *J.B.meta;
+  const uint32_t tc_shift = J.tc_shift;
+  const uint32_t y0 = (tile / J.tiles_x) << (6u - tc_shift);
+  const uint32_t x0 = (tile % J.tiles_x) << tc_shift;
+  double q[3];
+  uint32_t r_begin, r_end;
+  const Window win = block_cell_window(J.P, J.B, m, y0, x0, tc_shift, lane, q, r_begin, r_end);
+  uint32_t* out = J.plan + (size_t)tile * kPlanStride;
+  if (lane == 0) {
+    out[0] = r_begin;
+    out[1] = r_end;
+  }
+  for (uint32_t k = 0; k < kPlanRings; ++k) {
+    const uint32_t r = r_begin + k;
+    uint32_t lo = 0, hi = 0;
+    if (r_begin <= r_end && r <= r_end) {  // uniform
+      const uint32_t kr = r - m.firstring;
+      const uint32_t start = J.B.lut[kr], end = J.B.lut[kr + 1];
+      if (start != end) {
+        uint32_t lp, rp;
+        ring_cell_window(J.P, J.B, m, win, r, lp, rp);
+        ring_union_range(J.B, lp, rp, start, end, lo, hi);
+      }
+    }
+    if (lane == 0) {
+      out[2 + 2 * k] = lo;
+      out[3 + 2 * k] = hi;
+    }
+  }
+}
+
+int launch_mfma_plan(const MfmaJob& J, hipStream_t s) {
+  if (J.ntiles == 0) return HCG_OK;
+  k_mfma_plan<<<J.ntiles, 64, 0, s>>>(J);
+  HCG_CHECK_LAUNCH("k_mfma_plan");
+  return HCG_OK;
+}
+
 // LDS of one workgroup (the weight tiles as raw 16-bit storage).
 template <uint32_t CHUNK>
 struct MfmaLds {
@@ -102,9 +205,10 @@ struct MfmaLds {
   double wsum[kMcCols];
   double p[kMcCols][3];          // cell unit vectors
   uint32_t span[2];              // the block's ring span
-  uint32_t range[2][2];          // per ring parity: the union's sorted range
-  uint32_t lp[2][kMcCols];       // per ring parity: each cell's [lpix, rpix]
-  uint32_t rp[2][kMcCols];
+  uint32_t plan[kPlanStride];    // the tile's plan (when the job has one)
+  uint32_t range[2][2];          // the union's sorted range when searched here (double-buffered)
+  uint32_t lp[kMcCols];          // each cell's [lpix, rpix] in the current ring
+  uint32_t rp[kMcCols];
 };
 
 // CHUNK = staged samples per chunk.  LDS: two weight tiles of 64 x (CHUNK+8)
@@ -134,7 +238,8 @@ __device__ __forceinline__ void mfma_block(const MfmaJob& J, uint32_t tile, uint
   const GridParams& P = J.P;
   const GridBuffers& B = J.B;
   const uint16_t* __restrict__ vs = static_cast<const uint16_t*>(J.vs);
-  const uint32_t kt = J.kt, n_pad = J.n_pad, nchan = J.nchan, tc_shift = J.tc_shift, tiles_x = J.tiles_x;
+  const uint32_t kt = J.kt, nchan = J.nchan, tc_shift = J.tc_shift, tiles_x = J.tiles_x;
+  const size_t blk = (size_t)nchan * 32;  // elements per 32-sample block of the channel values
   float* __restrict__ cube = J.cube;
   float* __restrict__ wsum_out = J.wsum;
   const int normalize = J.normalize;
@@ -165,29 +270,19 @@ __device__ __forceinline__ void mfma_block(const MfmaJob& J, uint32_t tile, uint
   // other waves only stage, weigh and contract.
   Window win{};
   if (wave == 0) {
-    const uint32_t rl = y0 + (lane >> tc_shift), cl = x0 + (lane & tc_mask);
-    const bool lv = cl < P.side && rl < P.side;
-    const uint32_t celll = lv ? rl * P.side + cl : 0;
-    const uint32_t anchor = lv ? rl * P.side + (cl / P.factor) * P.factor : 0;
-    win = window_of(P, m.firstring, m.usedrings, B.xwcs[anchor], B.ywcs[anchor]);
-    if (!lv || m.usedrings == 0) {
-      win.uring = 1;
-      win.dring = 0;
-    }
-    double sb, cb0, sn, cs;
-    sincos(B.ywcs[celll] * kDeg2Rad, &sb, &cb0);
-    sincos(B.xwcs[celll] * kDeg2Rad, &sn, &cs);
-    L.p[lane][0] = cb0 * cs;
-    L.p[lane][1] = cb0 * sn;
-    L.p[lane][2] = sb;
-    const bool has = win.uring <= win.dring;
-    const uint32_t ur = wave_min_u32(has ? win.uring : 0xFFFFFFFFu);
-    const uint32_t dr = wave_max_u32(has ? win.dring : 0u);
+    double q[3];
+    uint32_t ur, dr;
+    win = block_cell_window(P, B, m, y0, x0, tc_shift, lane, q, ur, dr);
+    L.p[lane][0] = q[0];
+    L.p[lane][1] = q[1];
+    L.p[lane][2] = q[2];
     if (lane == 0) {
       L.span[0] = ur;
       L.span[1] = dr;
     }
   }
+  if (J.plan)
+    for (uint32_t i = tid; i < kPlanStride; i += nthreads) L.plan[i] = J.plan[(size_t)tile * kPlanStride + i];
   __syncthreads();
   const double px = L.p[c][0], py = L.p[c][1], pz = L.p[c][2];
   double wsum = 0.0;
@@ -206,128 +301,194 @@ __device__ __forceinline__ void mfma_block(const MfmaJob& J, uint32_t tile, uint
 #pragma unroll
   for (int nb = 0; nb < kMcNB; ++nb) {
     const uint32_t ch = ch_base + nb * 16 + (lane & 15);
-    bofs[nb] = (ch < nchan ? ch : nchan - 1);
+    bofs[nb] = (ch < nchan ? ch : nchan - 1) * 32u;
   }
 
-  uint32_t r_begin = L.span[0];
-  const uint32_t r_end = L.span[1];
-  if (r_begin == m.firstring) ++r_begin;  // the first used ring contributes nothing
+  const uint32_t r_begin = L.span[0], r_end = L.span[1];
 
-  for (uint32_t r = r_begin; r <= r_end && r_begin <= r_end; ++r) {
-    const uint32_t kr = r - m.firstring;
-    const uint32_t buf = r & 1u;  // ring-parity buffers: ring r+1 is written while ring r may still be read
-    if (wave == 0) {
-      const uint32_t start = B.lut[kr], end = B.lut[kr + 1];
-      uint32_t lo_r = 0, hi_r = 0;
-      if (start != end) {  // uniform
-        uint64_t a64 = 1, b64 = 0;
-        const bool act = r >= win.uring && r <= win.dring &&
-                         ring_window(P, win, r, B.ring_z[kr], B.ring_s[kr], a64, b64) && a64 <= b64;
-        const uint32_t lp = act ? (uint32_t)a64 : 1u, rp = act ? (uint32_t)b64 : 0u;
-        L.lp[buf][lane] = lp;  // (1, 0): no samples of this ring for the cell
-        L.rp[buf][lane] = rp;
-        const uint32_t LP = wave_min_u32(act ? lp : 0xFFFFFFFFu);
-        const uint32_t RP = wave_max_u32(act ? rp : 0u);
-        if (LP <= RP) {  // uniform
-          lo_r = wave_search<false>(B.keys, start, end, LP);
-          hi_r = wave_search<true>(B.keys, lo_r, end, RP);
-        }
+  // ---- per-chunk steps -------------------------------------------------
+  // stage records [c0 + from, c0 + cnt) of a chunk into LDS
+  auto stage = [&](uint32_t c0, uint32_t from, uint32_t cnt) {
+    for (uint32_t j = from + tid; j < cnt; j += nthreads) {
+      const hcg_sample q = B.rec[c0 + j];
+      s_x[j] = q.x;
+      s_y[j] = q.y;
+      s_z[j] = q.z;
+      s_key[j] = B.keys[c0 + j];
+    }
+  };
+  // zero the weight tiles' used width and the row blocks' k-block spans
+  auto clear = [&](uint32_t nkb) {
+    const uint32_t width8 = nkb * 4;  // 16-byte groups per row
+    for (uint32_t g = tid; g < kMcCols * width8; g += nthreads) {
+      const uint32_t rr = g / width8, cc = (g % width8) * 8;
+      *reinterpret_cast<uint4*>(&s_wh[rr * WS + cc]) = uint4{0u, 0u, 0u, 0u};
+      *reinterpret_cast<uint4*>(&s_wl[rr * WS + cc]) = uint4{0u, 0u, 0u, 0u};
+    }
+    if (tid < 4) {
+      s_mb[tid][0] = 0xFFFFFFFFu;
+      s_mb[tid][1] = 0u;
+    }
+  };
+  // weights of the chunk (after the staging barrier), then the contraction;
+  // pcnt > 0: the next chunk's first records [pc0, pc0 + min(pcnt, nthreads))
+  // are loaded during the contraction and written to LDS after it (the
+  // contraction reads only the weight tiles and the channel values)
+  auto weigh_and_contract = [&](uint32_t c0, uint32_t cnt, uint32_t nkb, uint32_t lp, uint32_t rp, uint32_t pc0,
+                                uint32_t pcnt) {
+    const bool act = lp <= rp;
+    // channel values of the chunk's first k-block, in flight while the
+    // weights are computed
+    vec bn[kMcNB];
+    if (ch_base < nchan) load_b_rows(bn, vs + (size_t)(c0 >> 5) * blk + koff, bofs);
+    // weights of this thread's cell over its band in the chunk
+    if (act) {
+      const uint32_t ja = lower_bound_u32(s_key, 0, cnt, lp);
+      const uint32_t jb = upper_bound_u32(s_key, ja, cnt, rp);
+      if (ja < jb && sub == 0) {
+        atomicMin(&s_mb[c >> 4][0], ja >> 5);
+        atomicMax(&s_mb[c >> 4][1], (jb + 31) >> 5);
       }
-      if (lane == 0) {
-        L.range[buf][0] = lo_r;
-        L.range[buf][1] = hi_r;
+      for (uint32_t j = ja + sub; j < jb; j += S) {
+        const double dx = px - s_x[j], dy = py - s_y[j], dz = pz - s_z[j];
+        const double c2 = fma(dx, dx, fma(dy, dy, dz * dz));
+        if (c2 < P.c_hi &&
+            (c2 < P.c_lo || radius_test_ref(B.xwcs, B.ywcs, B.lon_s, B.lat_s, cell, c0 + j, P.sphere_radius))) {
+          double wt = pair_weight_any(P, B, kt, c2, s_exp, cell, c0 + j);
+          if (B.w_s) wt *= B.w_s[c0 + j];
+          // the weight as hi + lo in the value type: ~2x its significand
+          const uint16_t hi = round16((float)wt, bf);
+          const float hif = widen16(hi, bf);
+          const uint16_t lo = round16((float)(wt - (double)hif), bf);
+          s_wh[c * WS + j] = hi;
+          s_wl[c * WS + j] = lo;
+          wsum += (double)hif + (double)widen16(lo, bf);
+        }
       }
     }
-    __syncthreads();
-    const uint32_t lo_u = L.range[buf][0], hi_u = L.range[buf][1];
-    if (lo_u >= hi_u) continue;  // uniform
-    const uint32_t lp = L.lp[buf][c], rp = L.rp[buf][c];
-    const bool act = lp <= rp;
-    for (uint32_t c0 = lo_u & ~7u; c0 < hi_u; c0 += CHUNK) {
-      const uint32_t cnt = min(CHUNK, hi_u - c0);
-      const uint32_t nkb = (cnt + 31) / 32;
-      __syncthreads();  // the previous chunk's weight tile is consumed
-      // stage the chunk; zero the weight tile's used width
-      for (uint32_t j = tid; j < cnt; j += nthreads) {
-        const hcg_sample q = B.rec[c0 + j];
-        s_x[j] = q.x;
-        s_y[j] = q.y;
-        s_z[j] = q.z;
-        s_key[j] = B.keys[c0 + j];
-      }
-      {
-        const uint32_t width8 = nkb * 4;  // 16-byte groups per row
-        for (uint32_t g = tid; g < kMcCols * width8; g += nthreads) {
-          const uint32_t rr = g / width8, cc = (g % width8) * 8;
-          *reinterpret_cast<uint4*>(&s_wh[rr * WS + cc]) = uint4{0u, 0u, 0u, 0u};
-          *reinterpret_cast<uint4*>(&s_wl[rr * WS + cc]) = uint4{0u, 0u, 0u, 0u};
-        }
-      }
-      if (tid < 4) {
-        s_mb[tid][0] = 0xFFFFFFFFu;
-        s_mb[tid][1] = 0u;
-      }
-      __syncthreads();
-      // channel values of the chunk's first k-block, in flight while the
-      // weights are computed
-      vec bn[kMcNB];
-      if (ch_base < nchan) load_b_rows(bn, vs + c0 + koff, bofs, n_pad);
-      // weights of this thread's cell over its band in the chunk
-      if (act) {
-        const uint32_t ja = lower_bound_u32(s_key, 0, cnt, lp);
-        const uint32_t jb = upper_bound_u32(s_key, ja, cnt, rp);
-        if (ja < jb && sub == 0) {
-          atomicMin(&s_mb[c >> 4][0], ja >> 5);
-          atomicMax(&s_mb[c >> 4][1], (jb + 31) >> 5);
-        }
-        for (uint32_t j = ja + sub; j < jb; j += S) {
-          const double dx = px - s_x[j], dy = py - s_y[j], dz = pz - s_z[j];
-          const double c2 = fma(dx, dx, fma(dy, dy, dz * dz));
-          if (c2 < P.c_hi &&
-              (c2 < P.c_lo || radius_test_ref(B.xwcs, B.ywcs, B.lon_s, B.lat_s, cell, c0 + j, P.sphere_radius))) {
-            double wt = pair_weight_any(P, B, kt, c2, s_exp, cell, c0 + j);
-            if (B.w_s) wt *= B.w_s[c0 + j];
-            // the weight as hi + lo in the value type: ~2x its significand
-            const uint16_t hi = round16((float)wt, bf);
-            const float hif = widen16(hi, bf);
-            const uint16_t lo = round16((float)(wt - (double)hif), bf);
-            s_wh[c * WS + j] = hi;
-            s_wl[c * WS + j] = lo;
-            wsum += (double)hif + (double)widen16(lo, bf);
+    __syncthreads();  // weights written; the staged records are no longer read
+    uint32_t mblo[4], mbhi[4];
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb) {
+      mblo[mb] = s_mb[mb][0];
+      mbhi[mb] = s_mb[mb][1];
+    }
+    const bool pre = tid < pcnt;
+    hcg_sample pq{};
+    uint32_t pkey = 0;
+    if (pre) {
+      pq = B.rec[pc0 + tid];
+      pkey = B.keys[pc0 + tid];
+    }
+    // contraction over the chunk's k-blocks
+    if (ch_base < nchan) {
+      // software pipeline: block u's MFMAs run while block u+1's channel
+      // values load
+      for (uint32_t u = 0; u < nkb; ++u) {
+        vec bc[kMcNB];
+#pragma unroll
+        for (int nb = 0; nb < kMcNB; ++nb) bc[nb] = bn[nb];
+        if (u + 1 < nkb) load_b_rows(bn, vs + (size_t)((c0 >> 5) + u + 1) * blk + koff, bofs);
+        // A: the row block's hi and lo weights, from LDS
+#pragma unroll
+        for (int mb = 0; mb < 4; ++mb) {
+          if (!(u >= mblo[mb] && u < mbhi[mb])) continue;
+          const uint32_t off = (mb * 16 + (lane & 15)) * WS + u * 32 + koff;
+          const vec ah = *reinterpret_cast<const vec*>(&s_wh[off]);
+          const vec al = *reinterpret_cast<const vec*>(&s_wl[off]);
+#pragma unroll
+          for (int nb = 0; nb < kMcNB; ++nb) {
+            acc[mb][nb] = mfma16(bf, ah, bc[nb], acc[mb][nb]);
+            acc[mb][nb] = mfma16(bf, al, bc[nb], acc[mb][nb]);
           }
         }
       }
-      __syncthreads();
-      // contraction over the chunk's k-blocks
-      uint32_t mblo[4], mbhi[4];
-#pragma unroll
-      for (int mb = 0; mb < 4; ++mb) {
-        mblo[mb] = s_mb[mb][0];
-        mbhi[mb] = s_mb[mb][1];
-      }
-      if (ch_base < nchan) {
-        // software pipeline: block u's MFMAs run while block u+1's channel
-        // values load
-        for (uint32_t u = 0; u < nkb; ++u) {
-          vec bc[kMcNB];
-#pragma unroll
-          for (int nb = 0; nb < kMcNB; ++nb) bc[nb] = bn[nb];
-          if (u + 1 < nkb) load_b_rows(bn, vs + c0 + (u + 1) * 32 + koff, bofs, n_pad);
-          // A: the row block's hi and lo weights, from LDS
-#pragma unroll
-          for (int mb = 0; mb < 4; ++mb) {
-            if (!(u >= mblo[mb] && u < mbhi[mb])) continue;
-            const uint32_t off = (mb * 16 + (lane & 15)) * WS + u * 32 + koff;
-            const vec ah = *reinterpret_cast<const vec*>(&s_wh[off]);
-            const vec al = *reinterpret_cast<const vec*>(&s_wl[off]);
-#pragma unroll
-            for (int nb = 0; nb < kMcNB; ++nb) {
-              acc[mb][nb] = mfma16(bf, ah, bc[nb], acc[mb][nb]);
-              acc[mb][nb] = mfma16(bf, al, bc[nb], acc[mb][nb]);
-            }
-          }
+    }
+    if (pre) {
+      s_x[tid] = pq.x;
+      s_y[tid] = pq.y;
+      s_z[tid] = pq.z;
+      s_key[tid] = pkey;
+    }
+  };
+
+  // ---- rings ----------------------------------------------------------------
+  auto planned = [&](uint32_t rr) { return J.plan != nullptr && (rr - r_begin) < kPlanRings; };
+  auto plan_lo = [&](uint32_t rr) { return L.plan[2 + 2 * (rr - r_begin)]; };
+  auto plan_hi = [&](uint32_t rr) { return L.plan[3 + 2 * (rr - r_begin)]; };
+  // the first ring at or after rr with samples, or past the plan
+  auto next_ring = [&](uint32_t rr) {
+    while (rr <= r_end && planned(rr) && plan_lo(rr) >= plan_hi(rr)) ++rr;
+    return rr;
+  };
+
+  uint32_t buf = 0;     // alternates per published range: a buffer is rewritten only after a later barrier
+  uint32_t staged = 0;  // records of the current chunk already in LDS (from the previous contraction)
+  for (uint32_t r = (r_begin <= r_end) ? next_ring(r_begin) : 1u; r <= r_end && r_begin <= r_end;
+       r = next_ring(r + 1)) {
+    const bool pl = planned(r);
+    uint32_t lo_u, hi_u;
+    if (pl) {
+      lo_u = plan_lo(r);
+      hi_u = plan_hi(r);
+    } else {
+      // past the plan: wave 0 finds the union's range itself and publishes
+      // it with the cell windows (nothing is staged ahead for such a ring)
+      const uint32_t kr = r - m.firstring;
+      buf ^= 1u;
+      if (wave == 0) {
+        uint32_t lo_r = 0, hi_r = 0;
+        const uint32_t start = B.lut[kr], end = B.lut[kr + 1];
+        if (start != end) {  // uniform
+          uint32_t a, b;
+          ring_cell_window(P, B, m, win, r, a, b);
+          L.lp[lane] = a;
+          L.rp[lane] = b;
+          ring_union_range(B, a, b, start, end, lo_r, hi_r);
+        }
+        if (lane == 0) {
+          L.range[buf][0] = lo_r;
+          L.range[buf][1] = hi_r;
         }
       }
+      __syncthreads();
+      lo_u = L.range[buf][0];
+      hi_u = L.range[buf][1];
+      if (lo_u >= hi_u) continue;  // uniform
+    }
+    uint32_t lp = 1u, rp = 0u;
+    for (uint32_t c0 = lo_u & ~31u; c0 < hi_u; c0 += CHUNK) {
+      const uint32_t cnt = min(CHUNK, hi_u - c0), nkb = (cnt + 31) / 32;
+      const bool first = c0 == (lo_u & ~31u);
+      __syncthreads();  // the previous chunk's weight tiles and the previous ring's windows are consumed
+      stage(c0, staged, cnt);
+      clear(nkb);
+      // a planned ring's cell windows: wave 0 computes them while the
+      // staging loads are in flight; they go out with the staging barrier
+      if (pl && first && wave == 0) {
+        uint32_t a, b;
+        ring_cell_window(P, B, m, win, r, a, b);
+        L.lp[lane] = a;
+        L.rp[lane] = b;
+      }
+      __syncthreads();
+      if (first) {
+        lp = L.lp[c];
+        rp = L.rp[c];
+      }
+      // the next chunk, staged during this contraction when its ring is planned
+      uint32_t nc0 = c0 + CHUNK, ncnt = 0;
+      if (nc0 < hi_u) {
+        ncnt = min(CHUNK, hi_u - nc0);
+      } else {
+        const uint32_t rn = next_ring(r + 1);
+        if (rn <= r_end && planned(rn)) {
+          nc0 = plan_lo(rn) & ~31u;
+          ncnt = min(CHUNK, plan_hi(rn) - nc0);
+        }
+      }
+      staged = min(ncnt, nthreads);
+      weigh_and_contract(c0, cnt, nkb, lp, rp, nc0, staged);
     }
   }
 
@@ -430,10 +591,16 @@ int launch_mfma_batch(uint32_t chunk, uint32_t waves, uint32_t blocks, const Mfm
   return HCG_OK;
 }
 
-// Channel gather: sorted, channel-major, zero-padded to n_pad.  A thread
-// owns one sorted slot and walks kGatherChans channels, so the permutation
-// is read once per kGatherChans channels rather than once per channel.
+// Channel gather into the blocked layout: sorted slot i of channel ch goes
+// to block i / 32, row ch, column i % 32 ([nblk][nchan][32], zero-padded to
+// nblk = hcg_channels_padded_len(n) / 32 blocks).  A thread owns one sorted
+// slot and walks kGatherChans channels, so the permutation is read once per
+// kGatherChans channels rather than once per channel.
 constexpr uint32_t kGatherChans = 32;
+
+__device__ inline size_t blocked_index(uint32_t i, uint32_t ch, uint32_t nchan) {
+  return ((size_t)(i >> 5) * nchan + ch) * 32u + (i & 31u);
+}
 
 template <typename T>
 __global__ void __launch_bounds__(256) k_gather_channels_cm(uint32_t n, uint32_t n_pad, uint32_t nchan,
@@ -456,7 +623,7 @@ __global__ void __launch_bounds__(256) k_gather_channels_cm(uint32_t n, uint32_t
       v = in[(size_t)ch * n + j];
       bad += isfinite((float)v) ? 0u : 1u;
     }
-    out[(size_t)ch * n_pad + i] = v;
+    out[blocked_index(i, ch, nchan)] = v;
   }
   if (nonfinite && bad) atomicAdd(nonfinite, bad);
 }
@@ -484,7 +651,7 @@ __global__ void __launch_bounds__(256) k_gather_channels_sm(uint32_t n, uint32_t
   __syncthreads();
   for (uint32_t k = ty; k < 64; k += 4) {
     const uint32_t ch = ch0 + k, i = i0 + tx;
-    if (ch < nchan && i < n_pad) out[(size_t)ch * n_pad + i] = tile[tx][k];
+    if (ch < nchan && i < n_pad) out[blocked_index(i, ch, nchan)] = tile[tx][k];
   }
 }
 
@@ -492,7 +659,7 @@ __global__ void __launch_bounds__(256) k_gather_channels_sm(uint32_t n, uint32_t
 
 using namespace hcg;
 
-extern "C" uint32_t hcg_channels_padded_len(uint32_t n) { return ((n + 7u) & ~7u) + 64u; }
+extern "C" uint32_t hcg_channels_padded_len(uint32_t n) { return ((n + 31u) & ~31u) + 64u; }
 
 extern "C" int hcg_gather_channels(uint32_t n, uint32_t nchan, const uint32_t* perm, const void* values, int dtype,
                                    int layout, void* values_sorted, uint32_t* nonfinite, void* stream) {
@@ -525,13 +692,21 @@ extern "C" int hcg_gather_channels(uint32_t n, uint32_t nchan, const uint32_t* p
   return HCG_OK;
 }
 
-extern "C" int hcg_grid_mfma_shaped(const hcg_healpix* hp, const hcg_kernel* kern, uint32_t side, uint32_t n,
-                                    uint32_t nchan, int dtype, const hcg_sample* records, const uint32_t* keys_sorted,
-                                    const double* lon_sorted, const double* lat_sorted, const double* weights_sorted,
-                                    const void* values_sorted, const uint32_t* lut, const double* ring_z,
-                                    const double* ring_s, const hcg_ring_meta* meta, const double* xwcs,
-                                    const double* ywcs, float* cube, float* wsum, int normalize,
-                                    const hcg_mfma_shape* shape, void* stream) {
+extern "C" size_t hcg_grid_mfma_workspace_size(uint32_t side, const hcg_mfma_shape* shape) {
+  const uint32_t tc = (shape && shape->tile_cols) ? shape->tile_cols : kMcDefaultTileCols;
+  if (!(tc == 8 || tc == 16 || tc == 32 || tc == 64)) return 0;
+  const size_t ntiles = (size_t)ceil_div_u32(side, tc) * ceil_div_u32(side, 64u / tc);
+  return ntiles * kPlanStride * sizeof(uint32_t);
+}
+
+extern "C" int hcg_grid_mfma_ex(const hcg_healpix* hp, const hcg_kernel* kern, uint32_t side, uint32_t n,
+                                uint32_t nchan, int dtype, const hcg_sample* records, const uint32_t* keys_sorted,
+                                const double* lon_sorted, const double* lat_sorted, const double* weights_sorted,
+                                const void* values_sorted, const uint32_t* lut, const double* ring_z,
+                                const double* ring_s, const hcg_ring_meta* meta, const double* xwcs,
+                                const double* ywcs, float* cube, float* wsum, int normalize,
+                                const hcg_mfma_shape* shape, void* workspace, size_t workspace_bytes,
+                                void* stream) {
   HCG_REQUIRE(hp && kern, "hcg_grid_mfma: NULL hp/kern");
   if (hp->nside == 0 || hp->nside > HCG_MAX_NSIDE)
     return set_error(HCG_ERR_UNSUPPORTED, "hcg_grid_mfma: nside must be <= 8192");
@@ -552,6 +727,7 @@ extern "C" int hcg_grid_mfma_shaped(const hcg_healpix* hp, const hcg_kernel* ker
   HCG_REQUIRE(records && keys_sorted && lon_sorted && lat_sorted && values_sorted && lut && ring_z && ring_s && meta &&
                   xwcs && ywcs && cube,
               "hcg_grid_mfma: NULL buffer");
+  hipStream_t s = as_stream(stream);
   GridParams P = make_grid_params(hp, kern, side);
   GridBuffers B{records, keys_sorted, lon_sorted, lat_sorted, weights_sorted, lut, ring_z, ring_s, meta,
                 xwcs,    ywcs,        nullptr,    nullptr};
@@ -572,12 +748,18 @@ extern "C" int hcg_grid_mfma_shaped(const hcg_healpix* hp, const hcg_kernel* ker
   const uint32_t tiles_x = ceil_div_u32(side, tc);
   const uint32_t ntiles = tiles_x * ceil_div_u32(side, tr);
   const dim3 grid(ntiles, ngroups);
-  hipStream_t s = as_stream(stream);
   // chunk size by workgroup width: 8 waves fill a CU alone (VGPR-limited);
   // narrower workgroups take smaller tiles so several share a CU
   const uint32_t chunk = sh.chunk ? sh.chunk : (nw >= 8 ? 512u : (nw >= 4 ? 256u : 128u));
-  MfmaJob J{P, B, values_sorted, cube, wsum, kern->kernel_type, n_pad, nchan, (uint32_t)dtype,
-            tc_shift, tiles_x, ntiles, (uint32_t)(normalize ? 1 : 0)};
+  MfmaJob J{P,        B,       values_sorted, cube,   wsum, kern->kernel_type, n_pad, nchan, (uint32_t)dtype,
+            tc_shift, tiles_x, ntiles,        (uint32_t)(normalize ? 1 : 0), nullptr};
+  if (workspace) {
+    if (workspace_bytes < (size_t)ntiles * kPlanStride * sizeof(uint32_t))
+      return set_error(HCG_ERR_WORKSPACE, "hcg_grid_mfma_ex: workspace smaller than hcg_grid_mfma_workspace_size");
+    J.plan = static_cast<uint32_t*>(workspace);
+    const int rc = launch_mfma_plan(J, s);
+    if (rc != HCG_OK) return rc;
+  }
 #define HCG_LAUNCH_MFMA(TT, CH) k_grid_mfma<TT, CH><<<grid, 64 * nw, 0, s>>>(J)
   if (dtype == HCG_DTYPE_BF16) {
     if (chunk == 512) HCG_LAUNCH_MFMA(kDtBf16, 512);
@@ -599,7 +781,19 @@ extern "C" int hcg_grid_mfma(const hcg_healpix* hp, const hcg_kernel* kern, uint
                              const void* values_sorted, const uint32_t* lut, const double* ring_z,
                              const double* ring_s, const hcg_ring_meta* meta, const double* xwcs, const double* ywcs,
                              float* cube, float* wsum, int normalize, void* stream) {
-  return hcg_grid_mfma_shaped(hp, kern, side, n, nchan, dtype, records, keys_sorted, lon_sorted, lat_sorted,
-                              weights_sorted, values_sorted, lut, ring_z, ring_s, meta, xwcs, ywcs, cube, wsum,
-                              normalize, nullptr, stream);
+  return hcg_grid_mfma_ex(hp, kern, side, n, nchan, dtype, records, keys_sorted, lon_sorted, lat_sorted,
+                          weights_sorted, values_sorted, lut, ring_z, ring_s, meta, xwcs, ywcs, cube, wsum, normalize,
+                          nullptr, nullptr, 0, stream);
+}
+
+extern "C" int hcg_grid_mfma_shaped(const hcg_healpix* hp, const hcg_kernel* kern, uint32_t side, uint32_t n,
+                                    uint32_t nchan, int dtype, const hcg_sample* records, const uint32_t* keys_sorted,
+                                    const double* lon_sorted, const double* lat_sorted, const double* weights_sorted,
+                                    const void* values_sorted, const uint32_t* lut, const double* ring_z,
+                                    const double* ring_s, const hcg_ring_meta* meta, const double* xwcs,
+                                    const double* ywcs, float* cube, float* wsum, int normalize,
+                                    const hcg_mfma_shape* shape, void* stream) {
+  return hcg_grid_mfma_ex(hp, kern, side, n, nchan, dtype, records, keys_sorted, lon_sorted, lat_sorted,
+                          weights_sorted, values_sorted, lut, ring_z, ring_s, meta, xwcs, ywcs, cube, wsum, normalize,
+                          shape, nullptr, 0, stream);
 }

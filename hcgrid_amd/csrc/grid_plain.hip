@@ -57,6 +57,12 @@
 #ifndef HCG_GRID_WAVES_PER_EU
 #define HCG_GRID_WAVES_PER_EU 4
 #endif
+#ifndef HCG_GRID_LANE_LOOP
+// 0: the wave walks the union of its lanes' bands (broadcast LDS reads; lanes
+// outside their band only evaluate the distance); 1: each lane walks its own
+// band (per-lane LDS addresses).  Measured on the 20M raster: 6.69 ms vs 7.14.
+#define HCG_GRID_LANE_LOOP 0
+#endif
 #ifndef HCG_GRID_TABLE_EXP
 #define HCG_GRID_TABLE_EXP 1
 #endif
@@ -178,6 +184,26 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(HCG_GRI
           je = max(je, b);
         }
       }
+#if HCG_GRID_LANE_LOOP
+      // each lane walks its own band: the trip count is the wave's longest
+      // band, not the bands' union (over half of which a lane would skip);
+      // per cell the samples are still taken in sorted order
+      uint32_t tmax = 0;
+#pragma unroll
+      for (int k = 0; k < kCellsPerLane; ++k) tmax = max(tmax, len[k]);
+      tmax = wave_max_u32(tmax);
+      (void)jb;
+      (void)je;
+      for (uint32_t t = 0; t < tmax; ++t) {
+#pragma unroll
+        for (int k = 0; k < kCellsPerLane; ++k) {
+          if (t >= len[k]) continue;
+          const uint32_t j = lo[k] + t;
+          const double qx = s_x[j], qy = s_y[j], qz = s_z[j], qv = s_v[j];
+          const double dx = px[k] - qx, dy = py[k] - qy, dz = pz[k] - qz;
+          const double c2 = fma(dx, dx, fma(dy, dy, dz * dz));
+          if (c2 < P.c_hi) {
+#else
       for (uint32_t j = jb; j < je; ++j) {
         const double qx = s_x[j], qy = s_y[j], qz = s_z[j], qv = s_v[j];
 #pragma unroll
@@ -185,6 +211,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(HCG_GRI
           const double dx = px[k] - qx, dy = py[k] - qy, dz = pz[k] - qz;
           const double c2 = fma(dx, dx, fma(dy, dy, dz * dz));
           if ((j - lo[k]) < len[k] && c2 < P.c_hi) {
+#endif
             if (c2 < P.c_lo || radius_test_ref(B.xwcs, B.ywcs, B.lon_s, B.lat_s, cell[k], cb + j, P.sphere_radius)) {
               double wt;
               if constexpr (KT == HCG_GAUSS1D)

@@ -14,6 +14,13 @@ constexpr int kMcNB = 4;                      // 16-channel blocks per wave
 constexpr int kMcMaxWaves = 8;
 constexpr uint32_t kMcChanPerWave = 16 * kMcNB;
 
+// Tile plan: per tile, its ring span and the sorted range of each ring's
+// union window, for up to kPlanRings rings (rings past that are searched in
+// the gridding kernel itself).  Layout per tile, kPlanStride uint32:
+// [r_begin, r_end, lo_0, hi_0, lo_1, hi_1, ...]; lo >= hi: no samples.
+constexpr uint32_t kPlanRings = 64;
+constexpr uint32_t kPlanStride = 2 + 2 * kPlanRings;
+
 // One multi-channel job: geometry, sorted channel values and outputs, with
 // its launch shape (tiles of (64 >> tc_shift) x (1 << tc_shift) cells).
 struct MfmaJob {
@@ -24,6 +31,7 @@ struct MfmaJob {
   float* wsum;        // [side * side] or NULL
   uint32_t kt, n_pad, nchan, dtype;
   uint32_t tc_shift, tiles_x, ntiles, normalize;
+  uint32_t* plan;     // [ntiles][kPlanStride], or NULL (search in the gridding kernel)
 };
 
 // One unit of the persistent launch: a tile of one job, one channel group.
@@ -34,6 +42,9 @@ struct MfmaItem {
 inline uint32_t tile_shift_of(uint32_t tile_cols) {
   return tile_cols == 8 ? 3u : tile_cols == 16 ? 4u : tile_cols == 32 ? 5u : 6u;
 }
+
+// Enqueues the tile-plan kernel of a job into J.plan (non-NULL).
+int launch_mfma_plan(const MfmaJob& J, hipStream_t s);
 
 // Grid size of the persistent kernel: CUs x resident workgroups per CU.
 int mfma_batch_blocks(uint32_t chunk, uint32_t waves, uint32_t* blocks);

@@ -206,7 +206,7 @@ class CubeGridder:
 
     Keys, sort, LUT and records as the single-channel path; channel values
     (bf16/f16, channel-major [nchan][n] or sample-major [n][nchan]) are
-    gathered into sorted channel-major rows and contracted on MFMA.
+    gathered into sorted order (blocked by 32 samples) and contracted on MFMA.
     """
 
     def __init__(self, n: int, side: int, nchan: int, kern: Kernel, hp: Healpix, dtype: str = "bf16",
@@ -228,19 +228,29 @@ class CubeGridder:
         self.cube = torch.empty((self.nchan, self.side * self.side), dtype=torch.float32, device=dev)
         self.wsum = torch.empty(self.side * self.side, dtype=torch.float32, device=dev)
         self.nonfinite = torch.zeros(1, dtype=torch.int32, device=dev)
+        self._ws = None
 
     def gather_channels(self, values, layout: int = 0, stream=None):
         call("hcg_gather_channels", self.n, self.nchan, _ptr(self.plain.buf.perm), _ptr(values), self.dcode, layout,
              _ptr(self.values_sorted), _ptr(self.nonfinite), _stream_handle(stream))
 
-    def grid(self, xwcs, ywcs, normalize: bool = True, stream=None, shape=None):
-        """shape: None (library defaults) or (tile_cols, waves, chunk), 0 = default."""
+    def grid(self, xwcs, ywcs, normalize: bool = True, stream=None, shape=None, plan: bool = True):
+        """shape: None (library defaults) or (tile_cols, waves, chunk), 0 = default.
+        plan: precompute the tile plan in a workspace (hcg_grid_mfma_ex)."""
+        torch = _torch()
         b = self.plain.buf
-        sh = None if shape is None else C.byref(capi.MfmaShape(*shape, 0))
-        call("hcg_grid_mfma_shaped", C.byref(self.hp), C.byref(self.kern), self.side, self.n, self.nchan,
+        shp = capi.MfmaShape(*(shape if shape is not None else (0, 0, 0)), 0)
+        ws, nb = None, 0
+        if plan:
+            nb = int(capi.load().hcg_grid_mfma_workspace_size(self.side, C.byref(shp)))
+            if self._ws is None or self._ws.numel() < nb:
+                self._ws = torch.empty(max(nb, 1), dtype=torch.uint8, device=self.plain.device)
+            ws = self._ws
+        call("hcg_grid_mfma_ex", C.byref(self.hp), C.byref(self.kern), self.side, self.n, self.nchan,
              self.dcode, _ptr(b.records), _ptr(b.keys), _ptr(b.lon_s), _ptr(b.lat_s), None,
              _ptr(self.values_sorted), _ptr(b.lut), _ptr(b.ring_z), _ptr(b.ring_s), _ptr(b.meta), _ptr(xwcs),
-             _ptr(ywcs), _ptr(self.cube), _ptr(self.wsum), 1 if normalize else 0, sh, _stream_handle(stream))
+             _ptr(ywcs), _ptr(self.cube), _ptr(self.wsum), 1 if normalize else 0, C.byref(shp), _ptr(ws), nb,
+             _stream_handle(stream))
 
     def gather_records(self, lon, lat, stream=None):
         """Sorted sample records and coordinates (the value slot stays 0)."""

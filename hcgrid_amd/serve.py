@@ -117,7 +117,8 @@ class GriddingServer:
         values = r.nchan * r.n * 2
         sorted_values = r.nchan * n_pad * 2 + 256
         cube = r.nchan * m * 4 + m * 4
-        plan = 4096 + 16 * (-(-r.side // 8) * -(-r.side // 8)) * max(1, -(-r.nchan // (64 * self.waves)))
+        tiles = -(-r.side // 8) * -(-r.side // 8)
+        plan = 4096 + tiles * 520 + 16 * tiles * max(1, -(-r.nchan // (64 * self.waves)))
         coords = 16 * r.n  # the request's lon/lat, alive until the round completes
         return values + sorted_values + cube + plan + coords
 

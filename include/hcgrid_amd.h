@@ -174,16 +174,19 @@ int hcg_grid_plain(const hcg_healpix *hp, const hcg_kernel *kern, uint32_t side,
 
 /* ---- multi-channel (spectral cube) path ---- */
 
-/* Row length (elements) of the sorted channel buffer: n rounded up to 8 plus
- * 64 zero elements, so the contraction can read whole 32-sample blocks. */
+/* Padded samples per channel of the sorted channel buffer: n rounded up to
+ * 32 plus 64 zero samples.  The buffer holds nchan * hcg_channels_padded_len(n)
+ * values, blocked by 32 sorted samples: [padded_len / 32][nchan][32], so the
+ * contraction reads each 32-sample block of 64 channels as one contiguous
+ * 4 KB piece. */
 uint32_t hcg_channels_padded_len(uint32_t n);
 
 /* The per-channel permutation gather of HCGrid+'s pre_order_data
  * (HCGrid+/gridding-omp.cu:146-157), all channels at once: values (dtype
  * HCG_DTYPE_BF16 or HCG_DTYPE_F16) laid out channel-major [nchan][n]
  * (layout 0, the reference's per-channel arrays) or sample-major
- * [n][nchan] (layout 1) -> values_sorted [nchan][hcg_channels_padded_len(n)]
- * in sorted sample order, zero-padded.  nonfinite (device, may be NULL)
+ * [n][nchan] (layout 1) -> values_sorted in the blocked layout above, in
+ * sorted sample order, zero-padded.  nonfinite (device, may be NULL)
  * is incremented once per non-finite value: the contraction multiplies
  * every staged value, so non-finite channel values are not supported. */
 int hcg_gather_channels(uint32_t n, uint32_t nchan, const uint32_t *perm, const void *values, int dtype,
@@ -218,8 +221,25 @@ typedef struct hcg_mfma_shape {
     uint32_t tile_cols, waves, chunk, reserved;
 } hcg_mfma_shape;
 
-/* hcg_grid_mfma with an explicit launch shape (NULL = defaults).  Results
- * do not depend on the shape beyond FP32 summation order. */
+/* Workspace bytes of hcg_grid_mfma_ex for a side x side map and a launch
+ * shape (NULL = defaults): the tile plan, 520 bytes per 64-cell tile. */
+size_t hcg_grid_mfma_workspace_size(uint32_t side, const hcg_mfma_shape *shape);
+
+/* hcg_grid_mfma with a launch shape (NULL = defaults) and a workspace for
+ * the tile plan (NULL = none).  With a workspace, a first small kernel (one
+ * wave per tile, many per CU) computes every tile's ring span and each
+ * ring's sorted range, and the gridding kernel reads them instead of
+ * searching the keys itself; the results are the same either way. */
+int hcg_grid_mfma_ex(const hcg_healpix *hp, const hcg_kernel *kern, uint32_t side, uint32_t n,
+                     uint32_t nchan, int dtype, const hcg_sample *records, const uint32_t *keys_sorted,
+                     const double *lon_sorted, const double *lat_sorted, const double *weights_sorted,
+                     const void *values_sorted, const uint32_t *lut, const double *ring_z,
+                     const double *ring_s, const hcg_ring_meta *meta, const double *xwcs,
+                     const double *ywcs, float *cube, float *wsum, int normalize,
+                     const hcg_mfma_shape *shape, void *workspace, size_t workspace_bytes, void *stream);
+
+/* hcg_grid_mfma with an explicit launch shape (NULL = defaults), no tile
+ * plan.  Results do not depend on the shape beyond FP32 summation order. */
 int hcg_grid_mfma_shaped(const hcg_healpix *hp, const hcg_kernel *kern, uint32_t side, uint32_t n,
                          uint32_t nchan, int dtype, const hcg_sample *records, const uint32_t *keys_sorted,
                          const double *lon_sorted, const double *lat_sorted, const double *weights_sorted,
@@ -316,7 +336,8 @@ typedef struct hcg_batch_job {
 
 typedef struct hcg_batch hcg_batch;   /* opaque */
 
-/* Device bytes a job adds to its batch (its sorted channel values). */
+/* Device bytes of a job's sorted channel values in its batch (the batch
+ * adds the job's tile plan, 520 bytes per 64-cell tile, and its work items). */
 size_t hcg_batch_job_bytes(const hcg_geometry *g, uint32_t nchan);
 
 /* Plans a batch: orders the work items (tile x channel group of every job,

@@ -160,3 +160,37 @@ def test_cube_full_size_band(gpu):
     print(f"  errors {e} wsum_rel_max {wrel.max():.2e}", flush=True)
     assert e["nan_match"] and e["max_rel"] <= 1e-4 and e["rms_rel"] <= 1e-4
     assert wrel.max() <= 2.0 ** -14
+
+
+@pytest.mark.parametrize("sigma,radius,shape", [(0.02, 0.06, (16, 0, 0)), (0.1, 0.3, (8, 4, 256)),
+                                                (0.1, 0.3, (64, 2, 128))])
+def test_cube_tile_plan_equals_search(gpu, sigma, radius, shape):
+    # the tile plan holds 64 rings per tile; sigma 0.1 deg at nside 8192 spans
+    # ~70-120 rings per tile, so planned and searched rings mix
+    import torch
+    wl = synth.raster_small(40, 200, 24, seed=31)
+    t = wl.target
+    xw, yw = gridder.wcs_sin(24, 24, t.crpix, t.crpix, -t.cdelt, t.cdelt, t.crval1, t.crval2)
+    V = _values(wl, 40, 31)
+    Vt = torch.tensor(V).to(torch.bfloat16).cuda()
+    kern, hp = gridder.prepare_grid_kernel(1, [sigma], radius, 0.01)
+    g = gridder.CubeGridder(wl.n, 24, 40, kern, hp)
+    d = lambda a: torch.as_tensor(a, dtype=torch.float64, device="cuda")  # noqa: E731
+    xd, yd = d(xw), d(yw)
+    print(f"case: plan vs search sigma={sigma} shape={shape} nside={hp.nside}", flush=True)
+    g.run(d(wl.lon), d(wl.lat), Vt, xd, yd, shape=shape)
+    torch.cuda.synchronize()
+    planned = g.cube.cpu().numpy().copy()
+    g.grid(xd, yd, shape=shape, plan=False)
+    torch.cuda.synchronize()
+    assert np.array_equal(planned, g.cube.cpu().numpy(), equal_nan=True)
+    Vr = Vt.float().cpu().numpy()
+    ref = oracle.pipeline(wl.lon, wl.lat, Vr[0], xw, yw, 24, sigma, radius, 0.01)
+    out, ws = oracle.grid_compat_multi(ref["h"], ref["g"], ref["lon_s"], ref["lat_s"], Vr[:, ref["perm"]], None,
+                                       ref["keys_sorted"], ref["firstring"], ref["usedrings"], ref["lut"], xw, yw,
+                                       0, 24)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        o = out / ws[None, :]
+    e = map_errors(planned.reshape(-1), o.reshape(-1))
+    print(f"  errors {e}", flush=True)
+    assert e["nan_match"] and e["max_rel"] <= 1e-4
