@@ -247,7 +247,7 @@ __device__ __forceinline__ void mfma_block(const MfmaJob& J, uint32_t tile, uint
   const hcg_ring_meta m = *B.meta;
   const uint32_t nthreads = blockDim.x;
   const uint32_t nw = nthreads >> 6;       // waves in this workgroup
-  const uint32_t S = nw;                   // threads per cell
+  const uint32_t S = nw;                   // threads per cell (one per wave)
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = __lane_id();
   const uint32_t wave = tid >> 6;
@@ -257,8 +257,10 @@ __device__ __forceinline__ void mfma_block(const MfmaJob& J, uint32_t tile, uint
   const uint32_t x0 = (tile % tiles_x) << tc_shift;
   const uint32_t M = P.side * P.side;
 
-  // this thread's cell: c = tid / S, sub = tid % S
-  const uint32_t c = tid / S, sub = tid % S;
+  // this thread's cell c = lane and its share sub = wave of the cell's band:
+  // every wave holds all 64 cells, so the weight work is balanced across
+  // the waves whichever rows of the block a ring reaches
+  const uint32_t c = lane, sub = wave;
   const uint32_t row = y0 + (c >> tc_shift);
   const uint32_t col = x0 + (c & tc_mask);
   const bool live = col < P.side && row < P.side;
@@ -492,11 +494,19 @@ __device__ __forceinline__ void mfma_block(const MfmaJob& J, uint32_t tile, uint
     }
   }
 
-  // per-cell weight sums: the S threads of a cell are consecutive lanes
-  for (uint32_t off = 1; off < S; off <<= 1) wsum += __shfl_xor(wsum, off, 64);
-  if (sub == 0) s_wsum[c] = wsum;
+  // per-cell weight sums over the waves (exact in FP64 in any order: every
+  // weight is hi + lo of two 16-bit values); the weight tiles are free now
   __syncthreads();
-  if (group == 0 && wsum_out && sub == 0 && live) wsum_out[cell] = (float)wsum;
+  double* wpart = reinterpret_cast<double*>(L.wh);
+  wpart[wave * kMcCols + lane] = wsum;
+  __syncthreads();
+  if (wave == 0) {
+    double t = 0.0;
+    for (uint32_t w = 0; w < nw; ++w) t += wpart[w * kMcCols + lane];
+    s_wsum[lane] = t;
+    if (group == 0 && wsum_out && live) wsum_out[cell] = (float)t;
+  }
+  __syncthreads();
   if (ch_base < nchan) {
 #pragma unroll
     for (int mb = 0; mb < 4; ++mb) {
