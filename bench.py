@@ -382,7 +382,9 @@ def bench_served(args, log):
     stream = serve.request_stream(args.served_jobs, rate=20.0, seed=4, repeat_frac=0.3, recent=32)
     srv = serve.GriddingServer(budget_bytes=200e9, slots=2)
     t0 = time.perf_counter()
-    done = serve.run_load(srv, stream)
+    done = serve.run_load(srv, stream,
+                          progress=lambda k, n: log(f"served: {k}/{n} requests done, "
+                                                    f"{time.perf_counter() - t0:.0f}s"))
     wall = time.perf_counter() - t0
     summ = serve.latency_summary(done)
     stats = dict(srv.stats)

@@ -233,10 +233,15 @@ class GriddingServer:
         return done
 
     # -- main loop -----------------------------------------------------------------
-    def serve(self, n_total: int, timeout_s: float = 3600.0):
-        """Runs until n_total requests have finished or been rejected."""
+    def serve(self, n_total: int, timeout_s: float = 3600.0, progress=None, every: int = 500):
+        """Runs until n_total requests have finished or been rejected;
+        progress(done, total) is called each time `every` more requests finish."""
         t_end = time.perf_counter() + timeout_s
+        next_report = every
         while len(self.finished) + len(self.rejected) < n_total:
+            if progress is not None and len(self.finished) >= next_report:
+                progress(len(self.finished), n_total)
+                next_report += every
             if time.perf_counter() > t_end:
                 raise TimeoutError(f"served {len(self.finished)} of {n_total} requests before the timeout")
             busy = False
@@ -301,7 +306,7 @@ def request_stream(n_requests: int, rate: float, seed: int = 4, repeat_frac: flo
     return out
 
 
-def run_load(server: GriddingServer, stream: list, time_scale: float = 1.0):
+def run_load(server: GriddingServer, stream: list, time_scale: float = 1.0, progress=None):
     """Submits the requests at their arrival offsets from a client thread
     while the server loop runs in this thread; returns the finished list."""
     t0 = time.perf_counter()
@@ -316,7 +321,7 @@ def run_load(server: GriddingServer, stream: list, time_scale: float = 1.0):
 
     th = threading.Thread(target=client, daemon=True)
     th.start()
-    done = server.serve(len(stream))
+    done = server.serve(len(stream), progress=progress)
     th.join()
     return done
 
