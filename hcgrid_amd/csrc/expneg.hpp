@@ -60,6 +60,23 @@ __device__ inline double exp_scaled_neg(double y, const double* lds_table) {
   return ldexp(fma(t, p, t), m);
 }
 
+// exp_scaled_neg for arguments known to be >= -92000 (no clamp): the same
+// operations, so the same results.
+__device__ inline double exp_scaled_neg_nc(double y, const double* lds_table) {
+  const double n = rint(y);
+  const int ni = (int)n;
+  const int j = ni & 63;
+  const int m = ni >> 6;
+  const double r = (y - n) * 0x1.62e42fefa39efp-7;
+  double q = fma(r, 1.0 / 120.0, 1.0 / 24.0);
+  q = fma(r, q, 1.0 / 6.0);
+  q = fma(r, q, 0.5);
+  q = fma(r, q, 1.0);
+  const double p = r * q;
+  const double t = lds_table[j];
+  return ldexp(fma(t, p, t), m);
+}
+
 __device__ inline double exp_neg(double x, const double* lds_table) {
   x = x < -1000.0 ? -1000.0 : x;
   const double n = rint(x * kExpInvLn2_64);

@@ -123,6 +123,20 @@ __device__ inline double gauss_exponent_scaled(const GridParams& P, double c2) {
   return P.k_asin * asin2_direct(0.25 * c2);
 }
 
+// The series branch of gauss_exponent_scaled alone, for grids whose radius
+// keeps every selected c2 below 4e-4 (R < ~1.1 deg): the same operations.
+__device__ inline double gauss_exponent_series(const GridParams& P, double c2) {
+  return c2 * fma(c2, fma(c2, fma(c2, P.k3, P.k2), P.k1), P.k0);
+}
+
+// True when every selected pair (c2 < c_hi) takes the series branch and its
+// scaled exponent stays above exp_scaled_neg's clamp (host side).
+inline bool gauss_small_radius(const GridParams& P) {
+  if (!(P.c_hi < 4e-4)) return false;
+  const double y = P.c_hi * (P.k0 + P.c_hi * (P.k1 + P.c_hi * (P.k2 + P.c_hi * P.k3)));
+  return y > -90000.0;
+}
+
 // GAUSS2D / TAPERED_SINC weights (HCGrid/gridding.cu:242-256).
 template <int KT>
 [[maybe_unused]] static __device__ __noinline__ double weight_general(double p0, double p1, double p2, const double* xwcs,
