@@ -326,12 +326,14 @@ int batch_init(uint32_t njobs, const hcg_batch_job* jobs, const BatchPlan& p, vo
     bt->values_sorted[j] = static_cast<char*>(bt->base) + p.voff[j];
     GridBuffers B{g->rec,  g->keys, g->lon_s, g->lat_s, nullptr, g->lut, g->ring_z,
                   g->ring_s, g->meta, g->xwcs,  g->ywcs,  nullptr, nullptr};
-    hj[j] = MfmaJob{make_grid_params(&g->hp, &g->kern, g->side),
+    const GridParams P = make_grid_params(&g->hp, &g->kern, g->side);
+    hj[j] = MfmaJob{P,
                     B,
                     bt->values_sorted[j],
                     b.cube,
                     b.wsum,
-                    g->kern.kernel_type,
+                    g->kern.kernel_type == HCG_GAUSS1D && gauss_small_radius(P) ? HCG_GAUSS1D + 16u
+                                                                                 : g->kern.kernel_type,
                     hcg_channels_padded_len(g->n),
                     b.nchan,
                     (uint32_t)b.dtype,

@@ -86,6 +86,9 @@ __device__ inline void load_b_rows(s16x8_t* b, const uint16_t* p, const uint32_t
 // FP64 weight of a selected pair (the single-channel kernel's evaluation).
 __device__ inline double pair_weight_any(const GridParams& P, const GridBuffers& B, uint32_t kt, double c2,
                                          const double* s_exp, uint32_t cell, uint32_t i) {
+  // kt == HCG_GAUSS1D + 16: GAUSS1D whose radius rules out the exponent's
+  // asin branch and the exp clamp (gauss_small_radius): the same values
+  if (kt == HCG_GAUSS1D + 16u) return exp_scaled_neg_nc(gauss_exponent_series(P, c2), s_exp);
   if (kt == HCG_GAUSS1D) return exp_scaled_neg(gauss_exponent_scaled(P, c2), s_exp);
   if (kt == HCG_GAUSS2D)
     return weight_general<HCG_GAUSS2D>(P.params[0], P.params[1], P.params[2], B.xwcs, B.ywcs, B.lon_s, B.lat_s,
@@ -748,8 +751,10 @@ extern "C" int hcg_grid_mfma_ex(const hcg_healpix* hp, const hcg_kernel* kern, u
   // with the weights
   uint32_t nw = sh.waves;
   if (nw == 0) {
+    // default: up to 4 waves (chunk 256), two workgroups per CU; measured
+    // faster than one 8-wave workgroup on the 2M x 1024 raster (16.9 vs 18.0 ms)
     nw = 1;
-    while (nw < groups_total && nw < (uint32_t)kMcMaxWaves) nw <<= 1;
+    while (nw < groups_total && nw < 4u) nw <<= 1;
   }
   const uint32_t ngroups = ceil_div_u32(groups_total, nw);
   const uint32_t tc = sh.tile_cols ? sh.tile_cols : kMcDefaultTileCols;
@@ -761,7 +766,8 @@ extern "C" int hcg_grid_mfma_ex(const hcg_healpix* hp, const hcg_kernel* kern, u
   // chunk size by workgroup width: 8 waves fill a CU alone (VGPR-limited);
   // narrower workgroups take smaller tiles so several share a CU
   const uint32_t chunk = sh.chunk ? sh.chunk : (nw >= 8 ? 512u : (nw >= 4 ? 256u : 128u));
-  MfmaJob J{P,        B,       values_sorted, cube,   wsum, kern->kernel_type, n_pad, nchan, (uint32_t)dtype,
+  const uint32_t ktc = kern->kernel_type == HCG_GAUSS1D && gauss_small_radius(P) ? HCG_GAUSS1D + 16u : kern->kernel_type;
+  MfmaJob J{P,        B,       values_sorted, cube,   wsum, ktc, n_pad, nchan, (uint32_t)dtype,
             tc_shift, tiles_x, ntiles,        (uint32_t)(normalize ? 1 : 0), nullptr};
   if (workspace) {
     if (workspace_bytes < (size_t)ntiles * kPlanStride * sizeof(uint32_t))

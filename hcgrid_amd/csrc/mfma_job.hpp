@@ -29,7 +29,7 @@ struct MfmaJob {
   const void* vs;     // sorted channel values [nchan][n_pad], BF16 or FP16
   float* cube;        // [nchan][side * side]
   float* wsum;        // [side * side] or NULL
-  uint32_t kt, n_pad, nchan, dtype;
+  uint32_t kt, n_pad, nchan, dtype;  // kt: kernel type; HCG_GAUSS1D + 16 = GAUSS1D at a small radius
   uint32_t tc_shift, tiles_x, ntiles, normalize;
   uint32_t* plan;     // [ntiles][kPlanStride], or NULL (search in the gridding kernel)
 };
