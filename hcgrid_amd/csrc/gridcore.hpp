@@ -20,6 +20,7 @@ struct GridParams {
   // GAUSS1D exponent in units of ln2/64: y = c2 (k0 + c2 (k1 + c2 (k2 + c2 k3))) for c2 < 4e-4
   // (the series of -p0 RAD2DEG^2 4 asin^2(sqrt(c2)/2) * 64/ln2), else k_asin * asin^2(sqrt(c2)/2)
   double k0, k1, k2, k3, k_asin;
+  double polar_cos, polar_sin, polar_half;  // cos R', sin R', sin(R'/2), R' = R (1 + 1e-6)
   uint32_t side;
   uint32_t factor;
 };
@@ -40,12 +41,51 @@ struct GridBuffers {
   double* wc;
 };
 
-// The window of an anchor cell (HCGrid/gridding.cu:306-323).
+// The window of an anchor cell (HCGrid/gridding.cu:306-323).  A polar window
+// takes whole rings; when tlo <= thi it is narrowed, in the polar-cap rings,
+// to the pixels whose longitude (in units of pi/2) lies in [tlo, thi]: a
+// range that provably holds every sample within the radius of the cell
+// (polar_phi_range), so the selected samples, and their order, are the same.
+struct PolarRange {
+  float tlo, thi;
+};
+
 struct Window {
   uint32_t uring, dring;
-  double phi;  // anchor longitude, radians
+  union {
+    double phi;        // not polar: anchor longitude, radians
+    PolarRange range;  // polar: [tlo, thi], or whole rings when tlo > thi
+  };
   bool polar;
 };
+
+// The longitude range of a polar window, in units of pi/2 (the HEALPix tt).
+// A sample at colatitude ts within distance d of a cell at colatitude tc
+// satisfies hav d = hav(tc - ts) + sin tc sin ts hav(dphi) >= sin tc sin ts
+// sin^2(dphi / 2), so d < R bounds |dphi| by 2 asin s, s = sin(R/2) /
+// sqrt(sin tc min sin ts), ts in (tc - R, tc + R) where sin is smallest at an
+// end: min sin ts = sin tc cos R - |cos tc| sin R.  asin s <= s / sqrt(1 - s^2)
+// (tan x >= x).  R is widened by 1e-6 relative on the host (the reference's
+// haversine decides the band around R to ~1e-15), the range by 1e-6 relative
+// and the pixel indices by one on each side (ring_window).  Only for factor
+// 1, where the anchor is the cell itself; no range (tlo > thi) when the disc
+// reaches a pole, s >= 1/2, or the range would wrap through longitude 0.
+__device__ inline void polar_phi_range(const GridParams& P, double b1, double l1, Window& w) {
+  w.range.tlo = 1.f;
+  w.range.thi = 0.f;
+  if (P.factor != 1u) return;
+  double sb, cb;
+  sincos(b1, &sb, &cb);  // sin tc = cos b1, cos tc = sin b1
+  const double smin = cb * P.polar_cos - fabs(sb) * P.polar_sin;
+  if (!(smin > 0.0)) return;
+  const double s = P.polar_half / sqrt(cb * smin);
+  if (!(s < 0.5)) return;
+  const double dt = (4.0 / kPi) * s / sqrt(1.0 - s * s) * (1.0 + 1e-6) + 1e-9;
+  const double tc = fmodulo4(l1 * kInvHalfPi);
+  if (!(tc - dt >= 0.0 && tc + dt < 4.0)) return;
+  w.range.tlo = (float)(tc - dt);
+  w.range.thi = (float)(tc + dt);
+}
 
 __device__ inline Window window_of(const GridParams& P, uint32_t firstring, uint32_t usedrings, double xa_deg,
                                    double ya_deg) {
@@ -64,6 +104,7 @@ __device__ inline Window window_of(const GridParams& P, uint32_t firstring, uint
   w.dring = (uint32_t)dring;
   w.phi = l1;
   w.polar = (disc_theta * kRad2Deg <= kNorthB) || (disc_theta * kRad2Deg >= kSouthB);
+  if (w.polar) polar_phi_range(P, b1, l1, w);
   return w;
 }
 
@@ -77,6 +118,18 @@ __device__ inline bool ring_window(const GridParams& P, const Window& w, uint64_
   if (w.polar) {
     lpix = sp;
     rpix = sp + npr - 1;
+    // in a polar-cap ring (ring < nside or > 3 nside; ir = npr / 4) the pixel
+    // of longitude tt is sp + floor(tt ir) (HCGrid/healpix.cu:126-205)
+    // (float: tt ir < 32768 is good to 0.004 pixel, inside the one-pixel margin)
+    if (w.range.tlo <= w.range.thi && (ring < P.hp.nside || ring > 3 * (uint64_t)P.hp.nside)) {
+      const float ir = (float)(uint32_t)(npr >> 2);
+      const int32_t lo = (int32_t)floorf(w.range.tlo * ir) - 1;
+      const int32_t hi = (int32_t)floorf(w.range.thi * ir) + 1;
+      if (lo >= 0 && hi < (int32_t)npr) {
+        lpix = sp + (uint64_t)lo;
+        rpix = sp + (uint64_t)hi;
+      }
+    }
     return true;
   }
   const bool have = rs >= 0.0;
@@ -235,6 +288,10 @@ inline GridParams make_grid_params(const hcg_healpix* hp, const hcg_kernel* kern
   P.k1 = P.k0 / 12.0;
   P.k2 = P.k0 / 90.0;
   P.k3 = P.k0 / 560.0;
+  const double rw = kern->sphere_radius * kDeg2Rad * (1.0 + 1e-6);
+  P.polar_cos = cos(rw);
+  P.polar_sin = sin(rw);
+  P.polar_half = sin(0.5 * rw);
   return P;
 }
 

@@ -83,9 +83,10 @@ def test_cube_many_channels_two_groups(gpu):
     _run_case(30, 150, 20, 600, "bf16", 0, seed=7)
 
 
-@pytest.mark.parametrize("dec", [-40.0, 75.0])
+@pytest.mark.parametrize("dec", [-78.0, -40.0, 75.0, 88.5])
 def test_cube_declinations(gpu, dec):
-    # 75 deg: the window anchors sit above NORTH_B, whole-ring windows
+    # |dec| >= 70 deg: the window anchors sit beyond NORTH_B / SOUTH_B,
+    # whole-ring windows narrowed to the disc's longitude range
     _run_case(40, 120, 24, 32, "bf16", 0, seed=11, dec=dec)
 
 

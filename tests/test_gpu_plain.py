@@ -71,9 +71,11 @@ def test_weights_and_f64_values(gpu):
     _compare(wl.lon, wl.lat, wl.values.astype(np.float64), xw, yw, wl.target.side, weights=w)
 
 
-@pytest.mark.parametrize("dec", [-55.0, 0.0, 62.0, 75.0, 85.0])
+@pytest.mark.parametrize("dec", [-78.0, -55.0, 0.0, 62.0, 75.0, 85.0, 88.5])
 def test_declinations(gpu, dec):
-    # 75 and 85 deg put the window anchor above NORTH_B: whole-ring windows
+    # |dec| >= 70 deg puts the window anchor above NORTH_B / below SOUTH_B:
+    # whole-ring windows, which the kernels narrow to the longitude range that
+    # holds the disc in polar-cap rings
     wl = synth.raster_small(40, 120, 24, dec_c=dec, spacing_ra=0.0032 / max(np.cos(np.radians(dec)), 0.1),
                             seed=13)
     xw, yw = _target_coords(wl.target)
@@ -86,6 +88,18 @@ def test_phi_wrap(gpu):
     wl.lon[:] = np.mod(wl.lon, 360.0)
     xw, yw = _target_coords(wl.target)
     _compare(wl.lon, wl.lat, wl.values, xw, yw, wl.target.side)
+
+
+@pytest.mark.parametrize("ra,factor,sigma,radius", [(0.2, 1, 0.02, 0.06), (180.0, 2, 0.02, 0.06),
+                                                     (180.0, 1, 0.1, 0.5)])
+def test_polar_window_fallbacks(gpu, ra, factor, sigma, radius):
+    # polar windows whose longitude range would wrap through 0, coarsening
+    # factor 2 (anchor != cell) and a large radius: whole rings where the
+    # narrowed range does not apply, the same map either way
+    wl = synth.raster_small(40, 160, 24, dec_c=80.0, ra_c=ra, spacing_ra=0.0032 / np.cos(np.radians(80.0)), seed=13)
+    wl.lon[:] = np.mod(wl.lon, 360.0)
+    xw, yw = _target_coords(wl.target)
+    _compare(wl.lon, wl.lat, wl.values, xw, yw, wl.target.side, sigma=sigma, radius=radius, factor=factor)
 
 
 @pytest.mark.parametrize("kt,params", [(GAUSS2D, [0.02, 0.015, 0.3]), (TAPERED_SINC, [0.02, 1.55, 2.52])])
