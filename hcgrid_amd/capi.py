@@ -22,6 +22,8 @@ HCG_ERR_IO = -6
 
 MAX_Y = 1073741824
 GAUSS1D, GAUSS2D, TAPERED_SINC = 1, 2, 3
+SELECT_COMPAT, SELECT_DISC = 0, 1
+SELECTIONS = {"compat": SELECT_COMPAT, "disc": SELECT_DISC}
 DTYPE_F64, DTYPE_F32, DTYPE_BF16, DTYPE_F16 = 0, 1, 2, 3
 
 
@@ -43,7 +45,7 @@ class Healpix(C.Structure):
 class Kernel(C.Structure):
     _fields_ = [
         ("kernel_type", C.c_uint32), ("bearing_needed", C.c_uint32),
-        ("factor", C.c_uint32), ("reserved", C.c_uint32),
+        ("factor", C.c_uint32), ("selection", C.c_uint32),
         ("params", C.c_double * 3), ("sphere_radius", C.c_double),
         ("disc_size", C.c_double),
     ]

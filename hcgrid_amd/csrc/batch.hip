@@ -135,6 +135,8 @@ int geometry_init(const hcg_healpix* hp, const hcg_kernel* kern, uint32_t n, uin
   HCG_REQUIRE(n < 0xFFFFFFFFu, "hcg_geometry_create: n must be < 2^32-1");
   HCG_REQUIRE(side <= 65535, "hcg_geometry_create: side must be <= 65535");
   HCG_REQUIRE(kern->factor >= 1, "hcg_geometry_create: factor must be >= 1");
+  HCG_REQUIRE(kern->selection <= HCG_SELECT_DISC,
+              "hcg_geometry_create: selection must be HCG_SELECT_COMPAT or HCG_SELECT_DISC");
   hcg_geometry* g = new (std::nothrow) hcg_geometry{};
   if (!g) return set_error(HCG_ERR_NOMEM, "hcg_geometry_create: host allocation failed");
   g->hp = *hp;

@@ -123,7 +123,7 @@ __device__ inline Window block_cell_window(const GridParams& P, const GridBuffer
   const bool has = win.uring <= win.dring;
   ur = wave_min_u32(has ? win.uring : 0xFFFFFFFFu);
   dr = wave_max_u32(has ? win.dring : 0u);
-  if (ur == m.firstring) ++ur;  // the first used ring contributes nothing (start_int - 1 wraps)
+  if (ur == m.firstring && skips_first_ring(P)) ++ur;  // the first used ring contributes nothing (start_int - 1 wraps)
   return win;
 }
 
@@ -726,6 +726,8 @@ extern "C" int hcg_grid_mfma_ex(const hcg_healpix* hp, const hcg_kernel* kern, u
   HCG_REQUIRE(kern->kernel_type >= HCG_GAUSS1D && kern->kernel_type <= HCG_TAPERED_SINC,
               "hcg_grid_mfma: unknown kernel type");
   HCG_REQUIRE(kern->factor >= 1, "hcg_grid_mfma: factor must be >= 1");
+  HCG_REQUIRE(kern->selection <= HCG_SELECT_DISC,
+              "hcg_grid_mfma: selection must be HCG_SELECT_COMPAT or HCG_SELECT_DISC");
   HCG_REQUIRE(dtype == HCG_DTYPE_BF16 || dtype == HCG_DTYPE_F16, "hcg_grid_mfma: dtype must be BF16 or F16");
   HCG_REQUIRE(side <= 65535, "hcg_grid_mfma: side must be <= 65535");
   hcg_mfma_shape sh = shape ? *shape : hcg_mfma_shape{0, 0, 0, 0};

@@ -138,7 +138,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(HCG_GRI
   }
   uint32_t r_begin = wave_min_u32(lane_ur);
   const uint32_t r_end = wave_max_u32(lane_dr);
-  if (r_begin == m.firstring) ++r_begin;  // the first used ring contributes nothing
+  if (r_begin == m.firstring && skips_first_ring(P)) ++r_begin;  // the first used ring contributes nothing
 
   for (uint32_t r = r_begin; r <= r_end && r_begin <= r_end; ++r) {
     const uint32_t kr = r - m.firstring;
@@ -313,7 +313,7 @@ __global__ void __launch_bounds__(64 * kRowWaves) k_grid_rows(GridParams P, Grid
     r_begin = min(r_begin, s_red[0][w][0]);
     r_end = max(r_end, s_red[0][w][1]);
   }
-  if (r_begin == m.firstring) ++r_begin;  // the first used ring contributes nothing
+  if (r_begin == m.firstring && skips_first_ring(P)) ++r_begin;  // the first used ring contributes nothing
   uint32_t phase = 1;                     // s_red[0] was just read; the next write goes to [1]
 
   for (uint32_t r = r_begin; r <= r_end && r_begin <= r_end; ++r) {
@@ -437,6 +437,8 @@ extern "C" int hcg_grid_plain(const hcg_healpix* hp, const hcg_kernel* kern, uin
   HCG_REQUIRE(kern->kernel_type >= HCG_GAUSS1D && kern->kernel_type <= HCG_TAPERED_SINC,
               "hcg_grid_plain: unknown kernel type");
   HCG_REQUIRE(kern->factor >= 1, "hcg_grid_plain: factor must be >= 1");
+  HCG_REQUIRE(kern->selection <= HCG_SELECT_DISC,
+              "hcg_grid_plain: selection must be HCG_SELECT_COMPAT or HCG_SELECT_DISC");
   HCG_REQUIRE(side <= 65535, "hcg_grid_plain: side must be <= 65535");
   if (side == 0) return HCG_OK;
   HCG_REQUIRE(records && keys_sorted && lon_sorted && lat_sorted && lut && ring_z && ring_s && meta && xwcs &&

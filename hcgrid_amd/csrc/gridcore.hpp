@@ -22,7 +22,8 @@ struct GridParams {
   double k0, k1, k2, k3, k_asin;
   double polar_cos, polar_sin, polar_half;  // cos R', sin R', sin(R'/2), R' = R (1 + 1e-6)
   uint32_t side;
-  uint32_t factor;
+  uint32_t factor;     // 1 under HCG_SELECT_DISC
+  uint32_t selection;  // HCG_SELECT_COMPAT / HCG_SELECT_DISC
 };
 
 struct GridBuffers {
@@ -41,11 +42,17 @@ struct GridBuffers {
   double* wc;
 };
 
-// The window of an anchor cell (HCGrid/gridding.cu:306-323).  A polar window
-// takes whole rings; when tlo <= thi it is narrowed, in the polar-cap rings,
-// to the pixels whose longitude (in units of pi/2) lies in [tlo, thi]: a
-// range that provably holds every sample within the radius of the cell
-// (polar_phi_range), so the selected samples, and their order, are the same.
+// The window of an anchor cell (HCGrid/gridding.cu:306-323).
+//  * kWinCompat: the reference's per-ring longitude window phi +- disc_size.
+//  * kWinPolar: the reference's whole rings (anchor colatitude <= 20 or >= 160
+//    deg), narrowed in the polar-cap rings to the pixels whose longitude (in
+//    units of pi/2) lies in [tlo, thi] when tlo <= thi: a range that provably
+//    holds every sample within the radius of the cell (phi_range), so the
+//    selected samples, and their order, are the reference's.
+//  * kWinDisc (HCG_SELECT_DISC): rings two beyond the disc's, each narrowed
+//    to [tlo, thi] when tlo <= thi, else whole: a superset of the disc.
+enum : uint32_t { kWinCompat = 0, kWinPolar = 1, kWinDisc = 2 };
+
 struct PolarRange {
   float tlo, thi;
 };
@@ -53,24 +60,25 @@ struct PolarRange {
 struct Window {
   uint32_t uring, dring;
   union {
-    double phi;        // not polar: anchor longitude, radians
-    PolarRange range;  // polar: [tlo, thi], or whole rings when tlo > thi
+    double phi;        // kWinCompat: anchor longitude, radians
+    PolarRange range;  // kWinPolar / kWinDisc: [tlo, thi], or whole rings when tlo > thi
   };
-  bool polar;
+  uint32_t mode;
 };
 
-// The longitude range of a polar window, in units of pi/2 (the HEALPix tt).
-// A sample at colatitude ts within distance d of a cell at colatitude tc
-// satisfies hav d = hav(tc - ts) + sin tc sin ts hav(dphi) >= sin tc sin ts
-// sin^2(dphi / 2), so d < R bounds |dphi| by 2 asin s, s = sin(R/2) /
-// sqrt(sin tc min sin ts), ts in (tc - R, tc + R) where sin is smallest at an
-// end: min sin ts = sin tc cos R - |cos tc| sin R.  asin s <= s / sqrt(1 - s^2)
-// (tan x >= x).  R is widened by 1e-6 relative on the host (the reference's
-// haversine decides the band around R to ~1e-15), the range by 1e-6 relative
-// and the pixel indices by one on each side (ring_window).  Only for factor
-// 1, where the anchor is the cell itself; no range (tlo > thi) when the disc
-// reaches a pole, s >= 1/2, or the range would wrap through longitude 0.
-__device__ inline void polar_phi_range(const GridParams& P, double b1, double l1, Window& w) {
+// The longitude range of the samples within the radius R of a cell, in units
+// of pi/2 (the HEALPix tt).  A sample at colatitude ts within distance d of a
+// cell at colatitude tc satisfies hav d = hav(tc - ts) + sin tc sin ts
+// hav(dphi) >= sin tc sin ts sin^2(dphi / 2), so d < R bounds |dphi| by
+// 2 asin s, s = sin(R/2) / sqrt(sin tc min sin ts), ts in (tc - R, tc + R)
+// where sin is smallest at an end: min sin ts = sin tc cos R - |cos tc| sin R.
+// asin s <= s / sqrt(1 - s^2) (tan x >= x).  R is widened by 1e-6 relative on
+// the host (the reference's haversine decides the band around R to ~1e-15),
+// the range by 1e-6 relative, and the pixel indices on each side (ring_window).
+// Valid only where the anchor is the cell (factor 1).  No range (tlo > thi)
+// when the disc reaches a pole, s >= 1/2, or the range would wrap through
+// longitude 0.
+__device__ inline void phi_range(const GridParams& P, double b1, double l1, Window& w) {
   w.range.tlo = 1.f;
   w.range.thi = 0.f;
   if (P.factor != 1u) return;
@@ -96,17 +104,32 @@ __device__ inline Window window_of(const GridParams& P, uint32_t firstring, uint
   double utheta = disc_theta - P.disc_size;
   if (utheta * kRad2Deg < 0) utheta = 0;
   uint64_t uring = pix2ring(P.hp, ang2pix(P.hp, utheta, l1));
-  if (uring < firstring) uring = firstring;
   utheta = disc_theta + P.disc_size;
+  if (P.selection == HCG_SELECT_DISC && utheta > kPi) utheta = kPi;  // the reference's MAX_Y key
   uint64_t dring = pix2ring(P.hp, ang2pix(P.hp, utheta, l1));
+  if (P.selection == HCG_SELECT_DISC) {
+    // a point lies in one of the two rings whose centres bracket it, so two
+    // rings either side cover every sample with |ts - tc| < R <= disc_size
+    uring = uring > 2 ? uring - 2 : 1;
+    dring += 2;
+    w.mode = kWinDisc;
+  } else {
+    w.mode = ((disc_theta * kRad2Deg <= kNorthB) || (disc_theta * kRad2Deg >= kSouthB)) ? kWinPolar : kWinCompat;
+  }
+  if (uring < firstring) uring = firstring;
   if (dring >= (uint64_t)firstring + usedrings) dring = (uint64_t)firstring + usedrings - 1;
   w.uring = (uint32_t)uring;
   w.dring = (uint32_t)dring;
-  w.phi = l1;
-  w.polar = (disc_theta * kRad2Deg <= kNorthB) || (disc_theta * kRad2Deg >= kSouthB);
-  if (w.polar) polar_phi_range(P, b1, l1, w);
+  if (w.mode == kWinCompat)
+    w.phi = l1;
+  else
+    phi_range(P, b1, l1, w);
   return w;
 }
+
+// The first used ring's samples are never gridded by the reference (the
+// start_int - 1 wrap, HCGrid/gridding.cu:326, 363-367); HCG_SELECT_DISC grids them.
+__device__ inline bool skips_first_ring(const GridParams& P) { return P.selection != HCG_SELECT_DISC; }
 
 // lpix/rpix of ring r (HCGrid/gridding.cu:336-360).  Returns false where the
 // reference would `continue` without advancing (an endless loop there); the
@@ -115,16 +138,21 @@ __device__ inline bool ring_window(const GridParams& P, const Window& w, uint64_
                                    uint64_t& lpix, uint64_t& rpix) {
   uint64_t sp, npr;
   ring_info(P.hp, ring, sp, npr);
-  if (w.polar) {
+  if (w.mode != kWinCompat) {
     lpix = sp;
     rpix = sp + npr - 1;
-    // in a polar-cap ring (ring < nside or > 3 nside; ir = npr / 4) the pixel
-    // of longitude tt is sp + floor(tt ir) (HCGrid/healpix.cu:126-205)
-    // (float: tt ir < 32768 is good to 0.004 pixel, inside the one-pixel margin)
-    if (w.range.tlo <= w.range.thi && (ring < P.hp.nside || ring > 3 * (uint64_t)P.hp.nside)) {
-      const float ir = (float)(uint32_t)(npr >> 2);
-      const int32_t lo = (int32_t)floorf(w.range.tlo * ir) - 1;
-      const int32_t hi = (int32_t)floorf(w.range.thi * ir) + 1;
+    // The pixel of longitude tt in ring r is sp + floor(tt npr / 4) in a
+    // polar-cap ring (HCGrid/healpix.cu:126-205), and that or the next (mod
+    // npr) in the others, whose pixels are diamonds.  Polar windows narrow
+    // the cap rings only (ring < nside or > 3 nside), with one pixel of
+    // margin; disc windows narrow every ring, with two.  (float: tt npr / 4
+    // < 32768 is good to 0.004 pixel.)
+    const bool cap = ring < P.hp.nside || ring > 3 * (uint64_t)P.hp.nside;
+    if (w.range.tlo <= w.range.thi && (cap || w.mode == kWinDisc)) {
+      const int32_t m = w.mode == kWinDisc ? 2 : 1;
+      const float q = (float)(uint32_t)(npr >> 2);
+      const int32_t lo = (int32_t)floorf(w.range.tlo * q) - m;
+      const int32_t hi = (int32_t)floorf(w.range.thi * q) + m;
       if (lo >= 0 && hi < (int32_t)npr) {
         lpix = sp + (uint64_t)lo;
         rpix = sp + (uint64_t)hi;
@@ -274,7 +302,8 @@ inline GridParams make_grid_params(const hcg_healpix* hp, const hcg_kernel* kern
   P.sphere_radius = kern->sphere_radius;
   for (int i = 0; i < 3; ++i) P.params[i] = kern->params[i];
   P.side = side;
-  P.factor = kern->factor;
+  P.selection = kern->selection;
+  P.factor = kern->selection == HCG_SELECT_DISC ? 1u : kern->factor;
   // radius threshold on c2 = 4 sin^2(R/2), with a 1e-9 relative guard band
   // where the reference's haversine decides (the unit-vector c2 is good to ~1e-12)
   const double half = kern->sphere_radius * kDeg2Rad / 2.0;

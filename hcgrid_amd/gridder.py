@@ -24,8 +24,12 @@ def _torch():
 
 
 def prepare_grid_kernel(kernel_type: int, kernel_params, sphere_radius: float, hpx_max_res: float,
-                        factor: int = 1):
-    """_prepare_grid_kernel (HCGrid/gmap.cpp:64-99) -> (Kernel, Healpix)."""
+                        factor: int = 1, selection: str = "compat"):
+    """_prepare_grid_kernel (HCGrid/gmap.cpp:64-99) -> (Kernel, Healpix).
+    selection: "compat" (the reference's windows) or "disc" (every sample
+    within sphere_radius; HCG_SELECT_DISC)."""
+    if selection not in capi.SELECTIONS:
+        raise capi.HcgError(capi.HCG_ERR_INVALID, "prepare_grid_kernel", f"selection {selection!r}")
     capi.load()
     params = (C.c_double * 3)(*([float(x) for x in kernel_params] + [0.0] * (3 - len(kernel_params))))
     kern = Kernel()
@@ -33,6 +37,7 @@ def prepare_grid_kernel(kernel_type: int, kernel_params, sphere_radius: float, h
     call("hcg_prepare_grid_kernel", kernel_type, params, float(sphere_radius), float(hpx_max_res),
          C.byref(kern), C.byref(hp))
     kern.factor = int(factor)
+    kern.selection = capi.SELECTIONS[selection]
     return kern, hp
 
 
@@ -179,12 +184,12 @@ class PlainGridder:
 
 def grid_plain(lon, lat, data, xwcs, ywcs, side: int, sigma: float, radius: float, hpx_res: float,
                weights=None, factor: int = 1, kernel_type: int = GAUSS1D, kernel_params=None,
-               device: int = 0) -> dict:
+               device: int = 0, selection: str = "compat") -> dict:
     """Host arrays in, host arrays out: one full device pass (for tests and tools)."""
     torch = _torch()
     if kernel_params is None:
         kernel_params = [sigma]
-    kern, hp = prepare_grid_kernel(kernel_type, kernel_params, radius, hpx_res, factor)
+    kern, hp = prepare_grid_kernel(kernel_type, kernel_params, radius, hpx_res, factor, selection)
     g = PlainGridder(len(lon), side, kern, hp, device=device, with_weights=weights is not None)
     dev = g.device
     t = lambda a, dt: torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device=dev)  # noqa: E731
@@ -269,11 +274,12 @@ class CubeGridder:
 
 
 def grid_cube(lon, lat, values_cm, xwcs, ywcs, side: int, sigma: float, radius: float, hpx_res: float,
-              dtype: str = "bf16", layout: int = 0, device: int = 0, shape=None) -> dict:
+              dtype: str = "bf16", layout: int = 0, device: int = 0, shape=None,
+              selection: str = "compat") -> dict:
     """Host arrays in/out: multi-channel pass (values_cm: torch tensor on the device).
     shape: None or (tile_cols, waves, chunk) for hcg_grid_mfma_shaped."""
     torch = _torch()
-    kern, hp = prepare_grid_kernel(GAUSS1D, [sigma], radius, hpx_res, 1)
+    kern, hp = prepare_grid_kernel(GAUSS1D, [sigma], radius, hpx_res, 1, selection)
     nchan = values_cm.shape[0] if layout == 0 else values_cm.shape[1]
     g = CubeGridder(len(lon), side, nchan, kern, hp, dtype=dtype, device=device)
     dev = g.plain.device

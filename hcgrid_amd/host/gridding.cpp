@@ -274,6 +274,7 @@ void solve_gridding(const char* infile, const char* tarfile, const char* outfile
   k.kernel_type = h_GMaps.kernel_type;
   k.bearing_needed = h_GMaps.bearing_needed ? 1 : 0;
   k.factor = h_GMaps.factor ? h_GMaps.factor : 1;
+  k.selection = h_GMaps.selection;
   for (int i = 0; i < 3; ++i) k.params[i] = h_kernel_params ? h_kernel_params[i] : 0.0;
   if (h_GMaps.kernel_type == GAUSS1D && h_kernel_params) {
     k.params[1] = 0.0;

@@ -10,11 +10,13 @@
 
 #include <string>
 
+#include "hcgrid_amd.h"
 #include "hcgrid_host.h"
 
 int main(int argc, char** argv) {
   char *path = nullptr, *ifile = nullptr, *tfile = nullptr, *ofile = nullptr, *sfile = nullptr, *num = nullptr,
-       *beam = nullptr, *order = nullptr, *bDim = nullptr, *Rg = nullptr, *Sp = nullptr, *factor = nullptr;
+       *beam = nullptr, *order = nullptr, *bDim = nullptr, *Rg = nullptr, *Sp = nullptr, *factor = nullptr,
+       *selection = nullptr;
   static const struct option long_options[] = {
       {"helparg", no_argument, nullptr, 'h'},          {"fits_path", required_argument, nullptr, 'p'},
       {"input_file", required_argument, nullptr, 'i'}, {"target_file", required_argument, nullptr, 't'},
@@ -22,7 +24,9 @@ int main(int argc, char** argv) {
       {"fits_id", required_argument, nullptr, 'n'},    {"beam_size", required_argument, nullptr, 'b'},
       {"order_arg", required_argument, nullptr, 'd'},  {"block_num", required_argument, nullptr, 'a'},
       {"register_num", required_argument, nullptr, 'r'}, {"sp_num", required_argument, nullptr, 'm'},
-      {"coarsening_factor", required_argument, nullptr, 'f'}, {nullptr, 0, nullptr, 0}};
+      {"coarsening_factor", required_argument, nullptr, 'f'},
+      // not in the reference: "compat" (default, the reference's windows) or "disc"
+      {"selection", required_argument, nullptr, 'S'}, {nullptr, 0, nullptr, 0}};
   int option_index = 0, c;
   while ((c = getopt_long_only(argc, argv, "hp:i:t:o:s:n:b:d:a:", long_options, &option_index)) != -1) {
     switch (c) {
@@ -44,6 +48,7 @@ int main(int argc, char** argv) {
       case 'r': Rg = optarg; break;
       case 'm': Sp = optarg; break;
       case 'f': factor = optarg; break;
+      case 'S': selection = optarg; break;
       default: return 1;
     }
   }
@@ -73,6 +78,15 @@ int main(int argc, char** argv) {
   h_GMaps.factor = 1;
   if (factor) h_GMaps.factor = atoi(factor);
   printf("h_GMaps.factor=%d, ", h_GMaps.factor);
+  h_GMaps.selection = HCG_SELECT_COMPAT;
+  if (selection) {
+    if (strcmp(selection, "disc") == 0) {
+      h_GMaps.selection = HCG_SELECT_DISC;
+    } else if (strcmp(selection, "compat") != 0) {
+      fprintf(stderr, "HCGrid: --selection must be compat or disc\n");
+      return 1;
+    }
+  }
   int block = 64;
   if (bDim) {
     block = atoi(bDim);

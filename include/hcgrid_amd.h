@@ -47,6 +47,9 @@ extern "C" {
 #define HCG_DTYPE_BF16     2
 #define HCG_DTYPE_F16      3
 #define HCG_MAX_NSIDE      8192u  /* keys are uint32; npix(8192) < MAX_Y, as the reference needs */
+/* hcg_kernel.selection: which (cell, sample) pairs are gridded */
+#define HCG_SELECT_COMPAT  0u  /* the reference's windows (HCGrid/gridding.cu:306-386), quirks included */
+#define HCG_SELECT_DISC    1u  /* every sample within sphere_radius of the cell, nothing else */
 
 /* HEALPix geometry: the fields of `struct Healpix` (HCGrid/healpix.h:14-21)
  * that the path reads.  Filled by hcg_healpix_init(). */
@@ -62,7 +65,11 @@ typedef struct hcg_kernel {
     uint32_t kernel_type;     /* HCG_GAUSS1D / HCG_GAUSS2D / HCG_TAPERED_SINC */
     uint32_t bearing_needed;  /* 1 for GAUSS2D */
     uint32_t factor;          /* coarsening factor gamma: cells per window anchor (>=1) */
-    uint32_t reserved;
+    uint32_t selection;       /* HCG_SELECT_COMPAT (default, 0) or HCG_SELECT_DISC; no counterpart
+                                 in the reference, whose selection is the compat one.  DISC takes
+                                 every sample with distance < sphere_radius: no first-ring skip, no
+                                 east-west truncation, windows across longitude 0 select both
+                                 sides; factor is then ignored (each cell is its own anchor) */
     double params[3];         /* GAUSS1D: params[0] = 0.5 / sigma^2 (sigma in degrees) */
     double sphere_radius;     /* degrees */
     double disc_size;         /* radians: DEG2RAD * sphere_radius + resolution */

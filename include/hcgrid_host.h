@@ -59,6 +59,7 @@ class GMaps {
     double disc_size;
     double sphere_radius;
     bool bearing_needed;
+    uint32_t selection;  /* not in the reference: HCG_SELECT_COMPAT (0, default) or HCG_SELECT_DISC */
 };
 extern GMaps h_GMaps;
 

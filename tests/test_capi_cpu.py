@@ -81,6 +81,19 @@ def test_prepare_grid_kernel_rejects_unknown_type():
     assert b"unknown kernel type" in lib.hcg_last_error_message()
 
 
+def test_selection_mode_plumbing():
+    kern, _ = gridder.prepare_grid_kernel(capi.GAUSS1D, [0.02], 0.06, 0.01)
+    assert kern.selection == capi.SELECT_COMPAT
+    kern, hp = gridder.prepare_grid_kernel(capi.GAUSS1D, [0.02], 0.06, 0.01, selection="disc")
+    assert kern.selection == capi.SELECT_DISC
+    with pytest.raises(capi.HcgError):
+        gridder.prepare_grid_kernel(capi.GAUSS1D, [0.02], 0.06, 0.01, selection="box")
+    lib = capi.load()
+    kern.selection = 2
+    assert lib.hcg_grid_plain(C.byref(hp), C.byref(kern), 4, 1, *([None] * 14)) == capi.HCG_ERR_INVALID
+    assert b"selection" in lib.hcg_last_error_message()
+
+
 def test_wcs_matches_oracle():
     for side, crval in ((90, (60.0, 30.0)), (400, (180.0, 30.0)), (64, (0.1, -45.0)), (50, (359.9, 80.0))):
         xw, yw = gridder.wcs_sin(side, side, side / 2, side / 2, -0.01, 0.01, *crval)

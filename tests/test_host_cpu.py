@@ -179,3 +179,7 @@ def test_cli_help_and_missing_args():
     assert r.returncode == 1 and "useage" in r.stderr
     r = subprocess.run([exe, "--fits_path", "/tmp/"], capture_output=True, text=True)
     assert r.returncode == 1 and "required" in r.stderr
+    args = ["--fits_path", "/nonexistent/", "--input_file", "in", "--target_file", "t", "--output_file", "o",
+            "--fits_id", "1", "--order_arg", "1"]
+    r = subprocess.run([exe, *args, "--selection", "box"], capture_output=True, text=True)
+    assert r.returncode == 1 and "--selection must be compat or disc" in r.stderr

@@ -136,6 +136,25 @@ def test_compat_equals_textbook_disc_near_equator(dec):
     assert np.allclose(r["weightscube"], wc, rtol=1e-12, atol=0)
 
 
+def test_textbook_disc_closed_forms():
+    # one sample per case at a known great-circle distance from one cell:
+    # across the pole (lon + 180, d = 2 (90 - dec)), across longitude 0, along
+    # a parallel at dec 60 (d = 2 asin(cos 60 sin(dlon / 2))), and just
+    # outside the radius (no contribution, NaN map)
+    p0 = 0.5 / 0.02 ** 2
+    cases = [((10.0, 89.99), (190.0, 89.99), 0.02), ((359.99, 0.0), (0.01, 0.0), 0.02),
+             ((100.0, 60.0), (100.05, 60.0), math.degrees(2 * math.asin(0.5 * math.sin(math.radians(0.025))))),
+             ((50.0, -30.0), (50.0, -30.0605), None)]
+    for (cx, cy), (sx, sy), d in cases:
+        dc, wc = O.grid_disc_textbook(np.array([sx]), np.array([sy]), np.array([2.0]), np.array([cx]),
+                                      np.array([cy]), 0.06, p0)
+        if d is None:
+            assert wc[0] == 0.0 and dc[0] == 0.0
+        else:
+            assert math.isclose(wc[0], math.exp(-d * d * p0), rel_tol=1e-12)
+            assert math.isclose(dc[0], 2.0 * wc[0], rel_tol=1e-15)
+
+
 def test_compat_window_truncates_east_west_at_high_dec():
     # At dec 60 the window is +-disc in longitude without 1/cos(dec), so it
     # holds only part of the disc: compat weights are strictly below the disc's.
