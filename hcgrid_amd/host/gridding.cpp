@@ -3,6 +3,7 @@
 // gfx950 one, and the host keeps the reference's module state up to date
 // (sorted h_lons/h_lats/h_data, h_hpx_idx with its npix sentinel,
 // h_start_ring, h_Healpix.firstring/usedrings).
+#include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -153,6 +154,141 @@ void presort_on_device(int sort_param) {
 
 }  // namespace
 
+namespace hcghost {
+
+hcg_kernel abi_kernel() {
+  hcg_kernel k{};
+  k.kernel_type = h_GMaps.kernel_type;
+  k.bearing_needed = h_GMaps.bearing_needed ? 1 : 0;
+  k.factor = h_GMaps.factor ? h_GMaps.factor : 1;
+  k.selection = h_GMaps.selection;
+  for (int i = 0; i < 3; ++i) k.params[i] = h_kernel_params ? h_kernel_params[i] : 0.0;
+  if (h_GMaps.kernel_type == GAUSS1D && h_kernel_params) {
+    k.params[1] = 0.0;
+    k.params[2] = 0.0;
+  }
+  k.sphere_radius = h_GMaps.sphere_radius;
+  k.disc_size = h_GMaps.disc_size;
+  return k;
+}
+
+bool device_perm(uint32_t* perm) {
+  if (!D.perm) return fail("device_perm: no pre-sort on the device");
+  return check_abi(hcg_memcpy_d2h(perm, D.perm, sizeof(uint32_t) * D.n, D.stream), "d2h") &&
+         check_abi(hcg_stream_synchronize(D.stream), "sync");
+}
+
+namespace {
+
+// Round to nearest even: double -> float -> bfloat16 (torch's path) or half.
+uint16_t to_bf16(double v) {
+  const float f = (float)v;
+  uint32_t x;
+  memcpy(&x, &f, 4);
+  if ((x & 0x7FFFFFFFu) > 0x7F800000u) return (uint16_t)((x >> 16) | 0x40);  // quiet NaN
+  return (uint16_t)((x + 0x7FFFu + ((x >> 16) & 1u)) >> 16);
+}
+
+uint16_t to_f16(double v) {
+  const float f = (float)v;
+  uint32_t x;
+  memcpy(&x, &f, 4);
+  const uint32_t sign = (x >> 16) & 0x8000u, ax = x & 0x7FFFFFFFu;
+  if (ax >= 0x7F800000u) return (uint16_t)(sign | (ax > 0x7F800000u ? 0x7E00u : 0x7C00u));
+  if (ax >= 0x477FF000u) return (uint16_t)(sign | 0x7C00u);  // 65520 and up round to infinity
+  if (ax < 0x38800000u) {                                    // below 2^-14: a multiple of 2^-24
+    float a;
+    memcpy(&a, &ax, 4);
+    return (uint16_t)(sign | (uint32_t)nearbyintf(a * 16777216.0f));
+  }
+  const uint32_t r = ax - 0x38000000u;  // exponent rebias 127 -> 15
+  return (uint16_t)(sign | ((r + 0xFFFu + ((r >> 13) & 1u)) >> 13));
+}
+
+}  // namespace
+
+bool grid_spectral_channels(const double* values, uint32_t nchan, int dtype, double* maps, double* wsum) {
+  if (!D.rec || !D.xwcs) return fail("grid_spectral_channels: pre-sort and data_alloc first");
+  if (dtype != HCG_DTYPE_F64 && dtype != HCG_DTYPE_BF16 && dtype != HCG_DTYPE_F16)
+    return fail("grid_spectral_channels: dtype must be F64, BF16 or F16");
+  const uint32_t n = D.n, side = h_zyx[1];
+  const uint64_t m = (uint64_t)h_zyx[1] * h_zyx[2];
+  const hcg_healpix hp = abi_healpix();
+  const hcg_kernel k = abi_kernel();
+  void* s = D.stream;
+  const double* w_s = D.unit_weights ? nullptr : D.w_s;
+  if (dtype == HCG_DTYPE_F64) {  // hcg_grid_plain per channel: the reference's FP64 arithmetic
+    double* dmap = nullptr;
+    if (!dalloc(dmap, m ? m : 1, "alloc")) return false;
+    bool ok = true;
+    for (uint32_t c = 0; ok && c < nchan; ++c) {
+      ok = check_abi(hcg_memcpy_h2d(D.data, values + (size_t)c * n, sizeof(double) * n, s), "h2d") &&
+           check_abi(hcg_gather_samples(n, D.perm, D.lon, D.lat, D.data, HCG_DTYPE_F64, nullptr, D.rec, nullptr,
+                                        nullptr, nullptr, nullptr, s),
+                     "hcg_gather_samples") &&
+           check_abi(hcg_memset(D.datacube, 0, sizeof(double) * m, s), "memset") &&
+           check_abi(hcg_memset(D.weightscube, 0, sizeof(double) * m, s), "memset") &&
+           check_abi(hcg_grid_plain(&hp, &k, side, n, D.rec, D.keys, D.lon_s, D.lat_s, w_s, D.lut, D.ring_z,
+                                    D.ring_s, D.meta, D.xwcs, D.ywcs, D.datacube, D.weightscube, s),
+                     "hcg_grid_plain") &&
+           check_abi(hcg_normalize(m, D.datacube, D.weightscube, dmap, s), "hcg_normalize") &&
+           check_abi(hcg_memcpy_d2h(maps + (size_t)c * m, dmap, sizeof(double) * m, s), "d2h") &&
+           (c != 0 || !wsum || check_abi(hcg_memcpy_d2h(wsum, D.weightscube, sizeof(double) * m, s), "d2h"));
+    }
+    ok = ok && check_abi(hcg_stream_synchronize(s), "sync");
+    hcg_free(dmap);
+    return ok;
+  }
+  // one multi-channel pass: values rounded to 16 bits, FP32 accumulation
+  if (nchan == 0) return true;
+  const size_t nv = (size_t)nchan * n;
+  const size_t np = (size_t)nchan * hcg_channels_padded_len(n);
+  uint16_t* hv = (uint16_t*)malloc(sizeof(uint16_t) * (nv ? nv : 1));
+  float* hc = (float*)malloc(sizeof(float) * (size_t)nchan * m);
+  float* hw = (float*)malloc(sizeof(float) * (m ? m : 1));
+  if (!hv || !hc || !hw) {
+    free(hv);
+    free(hc);
+    free(hw);
+    return fail("grid_spectral_channels: malloc() failed");
+  }
+  for (size_t i = 0; i < nv; ++i) hv[i] = dtype == HCG_DTYPE_BF16 ? to_bf16(values[i]) : to_f16(values[i]);
+  uint16_t *dv = nullptr, *dsorted = nullptr;
+  float *dcube = nullptr, *dws = nullptr;
+  uint32_t* dnf = nullptr;
+  uint32_t nonfinite = 0;
+  bool ok = dalloc(dv, nv ? nv : 1, "alloc") && dalloc(dsorted, np, "alloc") &&
+            dalloc(dcube, (size_t)nchan * m, "alloc") && dalloc(dws, m ? m : 1, "alloc") && dalloc(dnf, 1, "alloc");
+  ok = ok && check_abi(hcg_memcpy_h2d(dv, hv, sizeof(uint16_t) * nv, s), "h2d") &&
+       check_abi(hcg_memset(dnf, 0, sizeof(uint32_t), s), "memset") &&
+       check_abi(hcg_gather_channels(n, nchan, D.perm, dv, dtype, 0, dsorted, dnf, s), "hcg_gather_channels") &&
+       check_abi(hcg_grid_mfma(&hp, &k, side, n, nchan, dtype, D.rec, D.keys, D.lon_s, D.lat_s, w_s, dsorted, D.lut,
+                               D.ring_z, D.ring_s, D.meta, D.xwcs, D.ywcs, dcube, dws, 1, s),
+                 "hcg_grid_mfma") &&
+       check_abi(hcg_memcpy_d2h(hc, dcube, sizeof(float) * (size_t)nchan * m, s), "d2h") &&
+       check_abi(hcg_memcpy_d2h(hw, dws, sizeof(float) * m, s), "d2h") &&
+       check_abi(hcg_memcpy_d2h(&nonfinite, dnf, sizeof(uint32_t), s), "d2h") &&
+       check_abi(hcg_stream_synchronize(s), "sync");
+  if (ok && nonfinite) ok = fail("grid_spectral_channels: " + std::to_string(nonfinite) +
+                                 " non-finite channel values (the 16-bit path needs finite values)");
+  if (ok) {
+    for (size_t i = 0; i < (size_t)nchan * m; ++i) maps[i] = hc[i];
+    if (wsum)
+      for (uint64_t i = 0; i < m; ++i) wsum[i] = hw[i];
+  }
+  free(hv);
+  free(hc);
+  free(hw);
+  dfree(dv);
+  dfree(dsorted);
+  dfree(dcube);
+  dfree(dws);
+  dfree(dnf);
+  return ok;
+}
+
+}  // namespace hcghost
+
 void init_input_with_cpu(const int& sort_param) { presort_on_device(sort_param); }
 
 void init_input_with_thrust(const int& sort_param) {
@@ -270,18 +406,7 @@ void solve_gridding(const char* infile, const char* tarfile, const char* outfile
   printf("grid.x=%d, block.x=%d, ", gx, bx);
 
   const hcg_healpix hp = abi_healpix();
-  hcg_kernel k{};
-  k.kernel_type = h_GMaps.kernel_type;
-  k.bearing_needed = h_GMaps.bearing_needed ? 1 : 0;
-  k.factor = h_GMaps.factor ? h_GMaps.factor : 1;
-  k.selection = h_GMaps.selection;
-  for (int i = 0; i < 3; ++i) k.params[i] = h_kernel_params ? h_kernel_params[i] : 0.0;
-  if (h_GMaps.kernel_type == GAUSS1D && h_kernel_params) {
-    k.params[1] = 0.0;
-    k.params[2] = 0.0;
-  }
-  k.sphere_radius = h_GMaps.sphere_radius;
-  k.disc_size = h_GMaps.disc_size;
+  const hcg_kernel k = abi_kernel();
   const double tk = cpuSecond();
   if (!check_abi(hcg_grid_plain(&hp, &k, h_zyx[1], D.n, D.rec, D.keys, D.lon_s, D.lat_s,
                                 D.unit_weights ? nullptr : D.w_s, D.lut, D.ring_z, D.ring_s, D.meta, D.xwcs,

@@ -60,6 +60,9 @@ class GMaps {
     double sphere_radius;
     bool bearing_needed;
     uint32_t selection;  /* not in the reference: HCG_SELECT_COMPAT (0, default) or HCG_SELECT_DISC */
+    uint32_t spec_dtype; /* not in the reference: how the HCGrid+ path grids its channels,
+                            HCG_DTYPE_F64 (0, default: FP64 per channel, as HCGrid+) or
+                            HCG_DTYPE_BF16 / HCG_DTYPE_F16 (one multi-channel pass) */
 };
 extern GMaps h_GMaps;
 
@@ -105,6 +108,23 @@ void data_d2h();
 void data_free();
 void solve_gridding(const char *infile, const char *tarfile, const char *outfile, const char *sortfile,
                     const int &param, const int &bDim);
+
+/* HCGrid+'s spectral path (HCGrid+/gmap.h:126-128, HCGrid+/gridding.h): HDF5
+ * input (group coords: attribute num_samples, datasets xcoords, ycoords;
+ * group signal: datasets signal_0, signal_1, ...), one device pre-sort, every
+ * channel gridded.  The HDF5 reader is libhcgrid_h5.so next to this library,
+ * loaded on first use.  HCGrid+ reads exactly 5 channels; here spec_dim is
+ * the number of consecutive signal_<c> datasets. */
+void read_input_coordinate(const char *infile);  /* h_lons, h_lats, h_weights = 1, data_shape, spec_dim */
+void read_input_data(const char *infile);        /* h_spec_data */
+void solve_gridding(const char *infile, const char *tarfile, const char *outfile, const char *sortfile,
+                    const int &param, const int &bDim, int argc, char **argv);
+/* Not in the reference.  HCGrid+ keeps channels in double **h_data and
+ * writes only channel 0 (HCGrid+/gmap.cpp:354-361); here: */
+extern double *h_spec_data;  /* [spec_dim][data_shape] channel values, input order */
+extern double *h_spec_cube;  /* [spec_dim][NAXIS1*NAXIS2] normalised maps, NaN where no weight */
+void write_output_cube(const char *outfile);   /* every channel: NAXIS1 x NAXIS2 x spec_dim, BITPIX -64 */
+void write_ordered_cube(const char *outfile);  /* sorted input as FITS: 1 x N x spec_dim values + N x 2 coords */
 
 /* error-mode control, no counterpart in the reference */
 extern "C" void hcg_host_set_exit_on_error(int on);

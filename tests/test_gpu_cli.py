@@ -61,3 +61,78 @@ def test_cli_on_reference_fixtures(gpu, tmp_path, order):
     assert np.array_equal(sv, vals[perm].astype(np.float32).astype(np.float64))
     assert np.array_equal(sc[:n], lon[perm].astype(np.float32).astype(np.float64))
     assert np.array_equal(sc[n:], lat[perm].astype(np.float32).astype(np.float64))
+
+
+EXE_PLUS = os.path.join(ROOT, "hcgrid_amd", "HCGridPlus")
+
+
+def _write_h5(path, lon, lat, vals):
+    import ctypes as C
+    lib = os.path.join(ROOT, "hcgrid_amd", "libhcgrid_h5.so")
+    if not os.path.exists(lib):
+        pytest.skip("libhcgrid_h5.so not built")
+    L = C.CDLL(lib)
+    L.hcg_h5_write_input.argtypes = [C.c_char_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p,
+                                     C.c_int, C.c_char_p, C.c_size_t]
+    err = C.create_string_buffer(512)
+    v = np.ascontiguousarray(vals, dtype=np.float64)
+    assert L.hcg_h5_write_input(path.encode(), lon.size, lon.ctypes.data, lat.ctypes.data, v.shape[0],
+                                v.ctypes.data, 0, err, 512) == 0, err.value
+
+
+@pytest.mark.parametrize("cube_dtype", ["f64", "bf16"])
+def test_plus_cli_hdf5_cube(gpu, tmp_path, cube_dtype):
+    # HCGrid+'s flow on the reference's own field: the coordinates and values
+    # of input1.fits as channel 0 of a 5-channel HDF5 file, target1.fits as
+    # the target; every channel of the output cube against the oracle
+    import torch
+    shutil.copy(os.path.join(ROOT, "tests", "golden", "target1.fits"), tmp_path / "target1.fits")
+    _, vals = read_fits(os.path.join(ROOT, "tests", "golden", "input1.fits"), 0)
+    _, coords = read_fits(os.path.join(ROOT, "tests", "golden", "input1.fits"), 1)
+    n = vals.size
+    lon, lat = coords[:n].copy(), coords[n:].copy()
+    rng = np.random.default_rng(3)
+    V = np.stack([vals] + [vals * (1 + 0.5 * c) + rng.standard_normal(n) for c in range(1, 5)])
+    _write_h5(str(tmp_path / "input1.hdf5"), lon, lat, V)
+    d = str(tmp_path) + "/"
+    print(f"case: HCGridPlus cube_dtype={cube_dtype} n={n}", flush=True)
+    p = subprocess.run([EXE_PLUS, "--fits_path", d, "--input_file", "input", "--target_file", "target",
+                        "--output_file", "output", "--sorted_file", "sorted", "--fits_id", "1", "--beam_size", "300",
+                        "--order_arg", "2", "--cube_dtype", cube_dtype], capture_output=True, text=True, timeout=120)
+    print(p.stdout, p.stderr, flush=True)
+    assert p.returncode == 0
+    assert "channels=5" in p.stdout and "solving_gridding time=" in p.stdout
+    kv, m = read_fits(str(tmp_path / "output1.fits"), 0)
+    assert kv["BITPIX"] == "-64" and kv["NAXIS"] == "3" and kv["NAXIS3"] == "5"
+    tk, _ = read_fits(str(tmp_path / "target1.fits"), 0)
+    nx, ny = int(tk["NAXIS1"]), int(tk["NAXIS2"])
+    xw, yw = O.wcs_sin(nx, ny, float(tk["CRPIX1"]), float(tk["CRPIX2"]), float(tk["CDELT1"]),
+                       float(tk["CDELT2"]), float(tk["CRVAL1"]), float(tk["CRVAL2"]))
+    sigma = (300.0 / 3600.0) / math.sqrt(8 * math.log(2))
+    if cube_dtype == "bf16":  # both sides grid the values the device sees
+        Vg = torch.tensor(V.astype(np.float32)).to(torch.bfloat16).float().numpy().astype(np.float64)
+        tol = 1e-4
+    else:
+        Vg, tol = V, 1e-6
+    r = O.pipeline(lon, lat, Vg[0], xw, yw, ny, sigma, 5 * sigma, sigma / 2)
+    if cube_dtype == "bf16":  # 16-bit values are exact in the FP32 multi-channel restatement
+        out, ws = O.grid_compat_multi(r["h"], r["g"], r["lon_s"], r["lat_s"], Vg[:, r["perm"]], None,
+                                      r["keys_sorted"], r["firstring"], r["usedrings"], r["lut"], xw, yw, 0, ny)
+        maps = [normalise(out[c], ws) for c in range(5)]
+    else:  # the FP64 restatement per channel
+        maps = [normalise(r["datacube"], r["weightscube"])]
+        for c in range(1, 5):
+            rc = O.pipeline(lon, lat, Vg[c], xw, yw, ny, sigma, 5 * sigma, sigma / 2)
+            maps.append(normalise(rc["datacube"], rc["weightscube"]))
+    cube = m.reshape(5, nx * ny)
+    for c in range(5):
+        e = map_errors(cube[c], maps[c])
+        print(f"  channel {c}: {e}", flush=True)
+        assert e["nan_match"] and e["max_rel"] <= tol and e["cells"] > 1000
+    # the sorted input: 1 x N x 5 values and N x 2 coordinates in the stable order
+    skv, sv = read_fits(str(tmp_path / "sorted1.fits"), 0)
+    _, sc = read_fits(str(tmp_path / "sorted1.fits"), 1)
+    perm = r["perm"]
+    assert skv["NAXIS3"] == "5"
+    assert np.array_equal(sv.reshape(5, n), V[:, perm])
+    assert np.array_equal(sc[:n], lon[perm]) and np.array_equal(sc[n:], lat[perm])
