@@ -77,6 +77,41 @@ __device__ inline double exp_scaled_neg_nc(double y, const double* lds_table) {
   return ldexp(fma(t, p, t), m);
 }
 
+// a * b + c with c in a scalar register pair: one VOP3 v_fma_f64.  Written
+// out because the compiler otherwise keeps loop-invariant addends in vector
+// registers and emits the two-address v_fmac form plus a copy per call.
+__device__ inline double fma_vvs(double a, double b, double c) {
+  double d;
+  asm("v_fma_f64 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "s"(c));
+  return d;
+}
+
+// exp_scaled_neg_nc with ln2/64 folded into the polynomial: p = u (c1 + u (c2
+// + u (c3 + u (c4 + u c5)))), c_k = (ln2/64)^k / k!, u = y - rint(y), the same
+// degree-5 Taylor polynomial of exp(u ln2/64) - 1 without forming r, each
+// step one v_fma_f64 with its coefficient in scalar registers (the leading
+// pair as mul + add: two scalar operands cannot share one instruction).
+// For arguments >= -92000 (no clamp).
+__device__ inline double exp_scaled_neg_fold(double y, const double* lds_table) {
+  constexpr double c1 = 0x1.62e42fefa39efp-7;
+  constexpr double c2 = c1 * c1 / 2.0;
+  constexpr double c3 = c2 * c1 / 3.0;
+  constexpr double c4 = c3 * c1 / 4.0;
+  constexpr double c5 = c4 * c1 / 5.0;
+  const double n = rint(y);
+  const int ni = (int)n;
+  const int j = ni & 63;
+  const int m = ni >> 6;
+  const double u = y - n;
+  double q = u * c5 + c4;
+  q = fma_vvs(u, q, c3);
+  q = fma_vvs(u, q, c2);
+  q = fma_vvs(u, q, c1);
+  const double p = u * q;
+  const double t = lds_table[j];
+  return ldexp(fma(t, p, t), m);
+}
+
 __device__ inline double exp_neg(double x, const double* lds_table) {
   x = x < -1000.0 ? -1000.0 : x;
   const double n = rint(x * kExpInvLn2_64);

@@ -212,6 +212,7 @@ struct MfmaLds {
   uint32_t range[2][2];          // the union's sorted range when searched here (double-buffered)
   uint32_t lp[kMcCols];          // each cell's [lpix, rpix] in the current ring
   uint32_t rp[kMcCols];
+  Window win[kMcCols];           // each cell's window (read by wave 0 per ring)
 };
 
 // CHUNK = staged samples per chunk.  LDS: two weight tiles of 64 x (CHUNK+8)
@@ -269,15 +270,15 @@ __device__ __forceinline__ void mfma_block(const MfmaJob& J, uint32_t tile, uint
   const bool live = col < P.side && row < P.side;
   const uint32_t cell = live ? row * P.side + col : 0;
 
-  // Wave 0 holds the block's geometry, lane l = cell l: the cell's window
-  // and unit vector, the block's ring span, and per ring the cells'
-  // [lpix, rpix] and the union's sorted range, published through LDS.  The
-  // other waves only stage, weigh and contract.
-  Window win{};
+  // Wave 0 evaluates the block's geometry, lane l = cell l: the cell's
+  // window (kept in LDS, not registers, through the chunk loop) and unit
+  // vector, the block's ring span, and per ring the cells' [lpix, rpix] and
+  // the union's sorted range, published through LDS.  The other waves only
+  // stage, weigh and contract.
   if (wave == 0) {
     double q[3];
     uint32_t ur, dr;
-    win = block_cell_window(P, B, m, y0, x0, tc_shift, lane, q, ur, dr);
+    L.win[lane] = block_cell_window(P, B, m, y0, x0, tc_shift, lane, q, ur, dr);
     L.p[lane][0] = q[0];
     L.p[lane][1] = q[1];
     L.p[lane][2] = q[2];
@@ -446,7 +447,7 @@ __device__ __forceinline__ void mfma_block(const MfmaJob& J, uint32_t tile, uint
         const uint32_t start = B.lut[kr], end = B.lut[kr + 1];
         if (start != end) {  // uniform
           uint32_t a, b;
-          ring_cell_window(P, B, m, win, r, a, b);
+          ring_cell_window(P, B, m, L.win[lane], r, a, b);
           L.lp[lane] = a;
           L.rp[lane] = b;
           ring_union_range(B, a, b, start, end, lo_r, hi_r);
@@ -472,7 +473,7 @@ __device__ __forceinline__ void mfma_block(const MfmaJob& J, uint32_t tile, uint
       // staging loads are in flight; they go out with the staging barrier
       if (pl && first && wave == 0) {
         uint32_t a, b;
-        ring_cell_window(P, B, m, win, r, a, b);
+        ring_cell_window(P, B, m, L.win[lane], r, a, b);
         L.lp[lane] = a;
         L.rp[lane] = b;
       }

@@ -86,7 +86,8 @@ static_assert(kTileCols * kLaneRows == 64, "tile columns must divide 64");
 namespace hcg {
 
 
-template <int KT, bool HAS_W, bool SMALL_R>
+// WMODE (GAUSS1D): 0 general exponent and exp; 1 gauss_small_radius; 2 gauss_tiny_radius.
+template <int KT, bool HAS_W, int WMODE>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(HCG_GRID_WAVES_PER_EU))) k_grid_tiles(GridParams P, GridBuffers B, uint32_t tiles_x,
                                                    uint32_t tiles_y) {
   __shared__ double s_x[kStageCap], s_y[kStageCap], s_z[kStageCap], s_v[kStageCap];
@@ -206,6 +207,18 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(HCG_GRI
           const double dx = px[k] - qx, dy = py[k] - qy, dz = pz[k] - qz;
           const double c2 = fma(dx, dx, fma(dy, dy, dz * dz));
           if (c2 < P.c_hi) {
+#elif HCG_GRID_CELLS_PER_LANE == 1
+      // one cell per lane: its band is exactly [lo, lo + len)
+      (void)jb;
+      (void)je;
+      for (uint32_t t = 0; t < len[0]; ++t) {
+        const uint32_t j = lo[0] + t;
+        const double qx = s_x[j], qy = s_y[j], qz = s_z[j], qv = s_v[j];
+        {
+          constexpr int k = 0;
+          const double dx = px[k] - qx, dy = py[k] - qy, dz = pz[k] - qz;
+          const double c2 = fma(dx, dx, fma(dy, dy, dz * dz));
+          if (c2 < P.c_hi) {
 #else
       for (uint32_t j = jb; j < je; ++j) {
         const double qx = s_x[j], qy = s_y[j], qz = s_z[j], qv = s_v[j];
@@ -217,8 +230,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(HCG_GRI
 #endif
             if (c2 < P.c_lo || radius_test_ref(B.xwcs, B.ywcs, B.lon_s, B.lat_s, cell[k], cb + j, P.sphere_radius)) {
               double wt;
-              if constexpr (KT == HCG_GAUSS1D && SMALL_R)
-                wt = exp_scaled_neg_nc(gauss_exponent_series(P, c2), s_exp);
+              if constexpr (KT == HCG_GAUSS1D && WMODE > 0)
+                wt = gauss_weight_series<WMODE>(P, c2, s_exp);
               else if constexpr (KT == HCG_GAUSS1D)
                 wt = HCG_GRID_TABLE_EXP ? exp_scaled_neg(gauss_exponent_scaled(P, c2), s_exp)
                                         : exp(gauss_exponent_scaled(P, c2) * 0x1.62e42fefa39efp-7);
@@ -398,11 +411,13 @@ int launch_grid(const GridParams& P, const GridBuffers& B, hipStream_t s) {
     const uint64_t blocks = (uint64_t)tiles_x * tiles_y;
     if (blocks == 0) return HCG_OK;
     // GAUSS1D with a radius whose pairs all take the exponent's series branch
-    // (and stay clear of the exp clamp): the form without either check
-    if (KT == HCG_GAUSS1D && HCG_GRID_SMALLR && gauss_small_radius(P))
-      k_grid_tiles<KT, HAS_W, true><<<(uint32_t)blocks, 64, 0, s>>>(P, B, tiles_x, tiles_y);
+    // (and stay clear of the exp clamp): the forms without either check
+    if (KT == HCG_GAUSS1D && HCG_GRID_SMALLR && gauss_tiny_radius(P))
+      k_grid_tiles<KT, HAS_W, 2><<<(uint32_t)blocks, 64, 0, s>>>(P, B, tiles_x, tiles_y);
+    else if (KT == HCG_GAUSS1D && HCG_GRID_SMALLR && gauss_small_radius(P))
+      k_grid_tiles<KT, HAS_W, 1><<<(uint32_t)blocks, 64, 0, s>>>(P, B, tiles_x, tiles_y);
     else
-      k_grid_tiles<KT, HAS_W, false><<<(uint32_t)blocks, 64, 0, s>>>(P, B, tiles_x, tiles_y);
+      k_grid_tiles<KT, HAS_W, 0><<<(uint32_t)blocks, 64, 0, s>>>(P, B, tiles_x, tiles_y);
     HCG_CHECK_LAUNCH("k_grid_tiles");
   } else {
     const uint32_t segs_x = (P.side + 63) / 64;

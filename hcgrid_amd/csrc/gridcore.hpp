@@ -218,6 +218,23 @@ inline bool gauss_small_radius(const GridParams& P) {
   return y > -90000.0;
 }
 
+// gauss_small_radius and c_hi < 2e-5 (R < ~0.26 deg), where the series' c2^3
+// term, (4/560) c2^3 relative to the leading one, is below 6e-17 (host side).
+inline bool gauss_tiny_radius(const GridParams& P) { return gauss_small_radius(P) && P.c_hi < 2e-5; }
+
+// The GAUSS1D weight of a selected pair from the exponent's series (W = 1:
+// four terms, gauss_small_radius; W = 2: three terms, gauss_tiny_radius) and
+// exp_scaled_neg_fold; steps with a parameter as one v_fma_f64 each.
+template <int W>
+__device__ inline double gauss_weight_series(const GridParams& P, double c2, const double* s_exp) {
+  double t;
+  if constexpr (W == 2)
+    t = fma_vvs(c2, c2 * P.k2 + P.k1, P.k0);
+  else
+    t = fma_vvs(c2, fma_vvs(c2, c2 * P.k3 + P.k2, P.k1), P.k0);
+  return exp_scaled_neg_fold(c2 * t, s_exp);
+}
+
 // GAUSS2D / TAPERED_SINC weights (HCGrid/gridding.cu:242-256).
 template <int KT>
 [[maybe_unused]] static __device__ __noinline__ double weight_general(double p0, double p1, double p2, const double* xwcs,

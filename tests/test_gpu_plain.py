@@ -124,3 +124,12 @@ def test_larger_radius(gpu):
     wl = synth.raster_small(60, 300, 30, seed=29)
     xw, yw = _target_coords(wl.target)
     _compare(wl.lon, wl.lat, wl.values, xw, yw, wl.target.side, sigma=0.04, radius=0.2, res=0.02)
+
+
+@pytest.mark.parametrize("sigma,radius", [(0.15, 0.5), (0.4, 1.5)])
+def test_weight_paths_by_radius(gpu, sigma, radius):
+    # the GAUSS1D weight takes a 3-term exponent series below R ~ 0.26 deg (the
+    # other tests), a 4-term one up to ~1.1 deg (R 0.5) and asin beyond (R 1.5)
+    wl = synth.raster_small(40, 120, 24, spacing_dec=0.05, spacing_ra=0.01, cdelt=0.05, seed=31)
+    xw, yw = _target_coords(wl.target)
+    _compare(wl.lon, wl.lat, wl.values, xw, yw, wl.target.side, sigma=sigma, radius=radius, res=sigma / 2)
