@@ -209,10 +209,14 @@ struct MfmaLds {
   double p[kMcCols][3];          // cell unit vectors
   uint32_t span[2];              // the block's ring span
   uint32_t plan[kPlanStride];    // the tile's plan (when the job has one)
-  uint32_t range[2][2];          // the union's sorted range when searched here (double-buffered)
-  uint32_t lp[kMcCols];          // each cell's [lpix, rpix] in the current ring
-  uint32_t rp[kMcCols];
-  Window win[kMcCols];           // each cell's window (read by wave 0 per ring)
+  // chunk descriptors (double-buffered): blocks and segments; per block its
+  // sorted 32-sample block index, per segment its ring, first block, blocks
+  uint32_t dn[2][2];
+  uint32_t dgb[2][CHUNK / 32];
+  uint32_t dring[2][CHUNK / 32];
+  uint16_t du0[2][CHUNK / 32], dnb[2][CHUNK / 32];
+  uint32_t band[CHUNK / 32][kMcCols];  // each cell's band per segment: ja | jb << 16
+  Window win[kMcCols];                 // each cell's window
 };
 
 // CHUNK = staged samples per chunk.  LDS: two weight tiles of 64 x (CHUNK+8)
@@ -311,16 +315,89 @@ __device__ __forceinline__ void mfma_block(const MfmaJob& J, uint32_t tile, uint
   }
 
   const uint32_t r_begin = L.span[0], r_end = L.span[1];
+  const uint32_t n_all = m.n;       // sorted samples (records and keys beyond are not read)
+  constexpr uint32_t KB = CHUNK / 32;  // 32-sample blocks per chunk
+
+  auto planned = [&](uint32_t rr) { return J.plan != nullptr && (rr - r_begin) < kPlanRings; };
+  auto plan_lo = [&](uint32_t rr) { return L.plan[2 + 2 * (rr - r_begin)]; };
+  auto plan_hi = [&](uint32_t rr) { return L.plan[3 + 2 * (rr - r_begin)]; };
+  // sorted index of chunk position p of descriptor db
+  auto gidx = [&](uint32_t db, uint32_t p) { return L.dgb[db][p >> 5] * 32u + (p & 31u); };
+
+  // ---- chunk descriptors -----------------------------------------------------
+  // A chunk packs the 32-sample blocks of consecutive rings' union ranges,
+  // at most KB blocks: ring r's blocks [lo_r >> 5, (hi_r + 31) >> 5), in
+  // ascending ring and block order, so the staged keys are sorted and each
+  // cell's band in a ring is one interval of the chunk.  Wave 0 builds
+  // them, one ahead, walking the rings from its cursor (ring cr at block
+  // offset cbo of its range [clo, chi)); a ring past the plan has its
+  // range searched here (every lane of wave 0 takes part).
+  uint32_t cr = r_begin, cbo = 0, clo = 0, chi = 0;
+  bool crange = false;
+  auto build_desc = [&](uint32_t db) {  // wave 0, uniform
+    uint32_t nkb = 0, nseg = 0;
+    while (nkb < KB && r_begin <= r_end && cr <= r_end) {
+      if (!crange) {
+        clo = chi = 0;
+        if (planned(cr)) {
+          clo = plan_lo(cr);
+          chi = plan_hi(cr);
+        } else {
+          const uint32_t kr = cr - m.firstring;
+          const uint32_t start = B.lut[kr], end = B.lut[kr + 1];
+          if (start != end) {  // uniform
+            uint32_t a, b;
+            ring_cell_window(P, B, m, L.win[lane], cr, a, b);
+            ring_union_range(B, a, b, start, end, clo, chi);
+          }
+        }
+        crange = true;
+        cbo = 0;
+      }
+      if (clo >= chi) {
+        ++cr;
+        crange = false;
+        continue;
+      }
+      const uint32_t b0 = (clo >> 5) + cbo, b1 = (chi + 31) >> 5;
+      const uint32_t take = min(KB - nkb, b1 - b0);
+      if (lane == 0) {
+        for (uint32_t t = 0; t < take; ++t) L.dgb[db][nkb + t] = b0 + t;
+        L.dring[db][nseg] = cr;
+        L.du0[db][nseg] = (uint16_t)nkb;
+        L.dnb[db][nseg] = (uint16_t)take;
+      }
+      nkb += take;
+      ++nseg;
+      if (b0 + take == b1) {
+        ++cr;
+        crange = false;
+      } else {
+        cbo += take;
+      }
+    }
+    if (lane == 0) {
+      L.dn[db][0] = nkb;
+      L.dn[db][1] = nseg;
+    }
+  };
 
   // ---- per-chunk steps -------------------------------------------------
-  // stage records [c0 + from, c0 + cnt) of a chunk into LDS
-  auto stage = [&](uint32_t c0, uint32_t from, uint32_t cnt) {
-    for (uint32_t j = from + tid; j < cnt; j += nthreads) {
-      const hcg_sample q = B.rec[c0 + j];
-      s_x[j] = q.x;
-      s_y[j] = q.y;
-      s_z[j] = q.z;
-      s_key[j] = B.keys[c0 + j];
+  // stage chunk positions [from, 32 nkb) of descriptor db into LDS (keys past
+  // the samples are the empty key, beyond every window)
+  auto stage = [&](uint32_t db, uint32_t from, uint32_t nkb) {
+    for (uint32_t p = from + tid; p < nkb * 32u; p += nthreads) {
+      const uint32_t g = gidx(db, p);
+      if (g < n_all) {
+        const hcg_sample q = B.rec[g];
+        s_x[p] = q.x;
+        s_y[p] = q.y;
+        s_z[p] = q.z;
+        s_key[p] = B.keys[g];
+      } else {
+        s_x[p] = s_y[p] = s_z[p] = 0.0;
+        s_key[p] = 0xFFFFFFFFu;
+      }
     }
   };
   // zero the weight tiles' used width and the row blocks' k-block spans
@@ -336,32 +413,40 @@ __device__ __forceinline__ void mfma_block(const MfmaJob& J, uint32_t tile, uint
       s_mb[tid][1] = 0u;
     }
   };
-  // weights of the chunk (after the staging barrier), then the contraction;
-  // pcnt > 0: the next chunk's first records [pc0, pc0 + min(pcnt, nthreads))
-  // are loaded during the contraction and written to LDS after it (the
-  // contraction reads only the weight tiles and the channel values)
-  auto weigh_and_contract = [&](uint32_t c0, uint32_t cnt, uint32_t nkb, uint32_t lp, uint32_t rp, uint32_t pc0,
-                                uint32_t pcnt) {
-    const bool act = lp <= rp;
-    // channel values of the chunk's first k-block, in flight while the
-    // weights are computed
-    vec bn[kMcNB];
-    if (ch_base < nchan) load_b_rows(bn, vs + (size_t)(c0 >> 5) * blk + koff, bofs);
-    // weights of this thread's cell over its band in the chunk
-    if (act) {
-      const uint32_t ja = lower_bound_u32(s_key, 0, cnt, lp);
-      const uint32_t jb = upper_bound_u32(s_key, ja, cnt, rp);
-      if (ja < jb && sub == 0) {
-        atomicMin(&s_mb[c >> 4][0], ja >> 5);
-        atomicMax(&s_mb[c >> 4][1], (jb + 31) >> 5);
+  // each cell's band in each segment of the chunk (wave w: segments w, w +
+  // nw, ...), and the row blocks' k-block spans
+  auto bands = [&](uint32_t db, uint32_t nseg) {
+    for (uint32_t sg = wave; sg < nseg; sg += nw) {
+      uint32_t a, b;
+      ring_cell_window(P, B, m, L.win[lane], L.dring[db][sg], a, b);
+      uint32_t ja = 0, jb = 0;
+      if (a <= b) {
+        const uint32_t p0 = (uint32_t)L.du0[db][sg] * 32u, p1 = p0 + (uint32_t)L.dnb[db][sg] * 32u;
+        ja = lower_bound_u32(s_key, p0, p1, a);
+        jb = upper_bound_u32(s_key, ja, p1, b);
+        if (ja < jb) {
+          atomicMin(&s_mb[lane >> 4][0], ja >> 5);
+          atomicMax(&s_mb[lane >> 4][1], (jb + 31) >> 5);
+        }
       }
+      L.band[sg][lane] = ja | (jb << 16);
+    }
+  };
+  // weights of this thread's cell over its bands in the chunk (after the
+  // bands barrier)
+  auto weigh = [&](uint32_t db, uint32_t nseg) {
+    for (uint32_t sg = 0; sg < nseg; ++sg) {
+      const uint32_t bd = L.band[sg][c];
+      const uint32_t ja = bd & 0xFFFFu, jb = bd >> 16;
       for (uint32_t j = ja + sub; j < jb; j += S) {
         const double dx = px - s_x[j], dy = py - s_y[j], dz = pz - s_z[j];
         const double c2 = fma(dx, dx, fma(dy, dy, dz * dz));
         if (c2 < P.c_hi &&
-            (c2 < P.c_lo || radius_test_ref(B.xwcs, B.ywcs, B.lon_s, B.lat_s, cell, c0 + j, P.sphere_radius))) {
-          double wt = pair_weight_any(P, B, kt, c2, s_exp, cell, c0 + j);
-          if (B.w_s) wt *= B.w_s[c0 + j];
+            (c2 < P.c_lo || radius_test_ref(B.xwcs, B.ywcs, B.lon_s, B.lat_s, cell, gidx(db, j), P.sphere_radius))) {
+          // (the sample's sorted index only for the kernels that read its coordinates)
+          const bool need_i = kt == HCG_GAUSS2D || kt == HCG_TAPERED_SINC;
+          double wt = pair_weight_any(P, B, kt, c2, s_exp, cell, need_i ? gidx(db, j) : 0u);
+          if (B.w_s) wt *= B.w_s[gidx(db, j)];
           // the weight as hi + lo in the value type: ~2x its significand
           const uint16_t hi = round16((float)wt, bf);
           const float hif = widen16(hi, bf);
@@ -372,130 +457,81 @@ __device__ __forceinline__ void mfma_block(const MfmaJob& J, uint32_t tile, uint
         }
       }
     }
-    __syncthreads();  // weights written; the staged records are no longer read
+  };
+  // the contraction over the chunk's k-blocks; bn holds block 0's channel
+  // values on entry
+  auto contract = [&](uint32_t db, uint32_t nkb, vec* bn) {
     uint32_t mblo[4], mbhi[4];
 #pragma unroll
     for (int mb = 0; mb < 4; ++mb) {
       mblo[mb] = s_mb[mb][0];
       mbhi[mb] = s_mb[mb][1];
     }
-    const bool pre = tid < pcnt;
-    hcg_sample pq{};
-    uint32_t pkey = 0;
-    if (pre) {
-      pq = B.rec[pc0 + tid];
-      pkey = B.keys[pc0 + tid];
-    }
-    // contraction over the chunk's k-blocks
-    if (ch_base < nchan) {
-      // software pipeline: block u's MFMAs run while block u+1's channel
-      // values load
-      for (uint32_t u = 0; u < nkb; ++u) {
-        vec bc[kMcNB];
+    // software pipeline: block u's MFMAs run while block u+1's channel values load
+    for (uint32_t u = 0; u < nkb; ++u) {
+      vec bc[kMcNB];
 #pragma unroll
-        for (int nb = 0; nb < kMcNB; ++nb) bc[nb] = bn[nb];
-        if (u + 1 < nkb) load_b_rows(bn, vs + (size_t)((c0 >> 5) + u + 1) * blk + koff, bofs);
-        // A: the row block's hi and lo weights, from LDS
+      for (int nb = 0; nb < kMcNB; ++nb) bc[nb] = bn[nb];
+      if (u + 1 < nkb) load_b_rows(bn, vs + (size_t)L.dgb[db][u + 1] * blk + koff, bofs);
+      // A: the row block's hi and lo weights, from LDS
 #pragma unroll
-        for (int mb = 0; mb < 4; ++mb) {
-          if (!(u >= mblo[mb] && u < mbhi[mb])) continue;
-          const uint32_t off = (mb * 16 + (lane & 15)) * WS + u * 32 + koff;
-          const vec ah = *reinterpret_cast<const vec*>(&s_wh[off]);
-          const vec al = *reinterpret_cast<const vec*>(&s_wl[off]);
+      for (int mb = 0; mb < 4; ++mb) {
+        if (!(u >= mblo[mb] && u < mbhi[mb])) continue;
+        const uint32_t off = (mb * 16 + (lane & 15)) * WS + u * 32 + koff;
+        const vec ah = *reinterpret_cast<const vec*>(&s_wh[off]);
+        const vec al = *reinterpret_cast<const vec*>(&s_wl[off]);
 #pragma unroll
-          for (int nb = 0; nb < kMcNB; ++nb) {
-            acc[mb][nb] = mfma16(bf, ah, bc[nb], acc[mb][nb]);
-            acc[mb][nb] = mfma16(bf, al, bc[nb], acc[mb][nb]);
-          }
+        for (int nb = 0; nb < kMcNB; ++nb) {
+          acc[mb][nb] = mfma16(bf, ah, bc[nb], acc[mb][nb]);
+          acc[mb][nb] = mfma16(bf, al, bc[nb], acc[mb][nb]);
         }
       }
     }
+  };
+
+  // ---- chunks ---------------------------------------------------------------
+  // barriers per chunk: staging (B1), bands (B2), weights (B3), contraction
+  // done (T).  The next chunk's descriptor is built by wave 0 between B2 and
+  // B3; its first nthreads records load during the contraction and go to
+  // LDS after it (the contraction reads only weight tiles and channel values).
+  uint32_t db = 0, staged = 0;
+  if (wave == 0) build_desc(0);
+  __syncthreads();
+  for (;;) {
+    const uint32_t nkb = L.dn[db][0], nseg = L.dn[db][1];
+    if (nkb == 0) break;  // uniform
+    stage(db, staged, nkb);
+    clear(nkb);
+    __syncthreads();  // B1
+    bands(db, nseg);
+    __syncthreads();  // B2
+    // channel values of the chunk's first k-block, in flight while the weights are computed
+    vec bn[kMcNB];
+    if (ch_base < nchan) load_b_rows(bn, vs + (size_t)L.dgb[db][0] * blk + koff, bofs);
+    if (wave == 0) build_desc(db ^ 1u);
+    weigh(db, nseg);
+    __syncthreads();  // B3
+    const uint32_t nn = L.dn[db ^ 1u][0] * 32u;
+    staged = min(nn, nthreads);
+    const bool pre = tid < staged;
+    hcg_sample pq{};
+    uint32_t pkey = 0xFFFFFFFFu;
+    if (pre) {
+      const uint32_t g = gidx(db ^ 1u, tid);
+      if (g < n_all) {
+        pq = B.rec[g];
+        pkey = B.keys[g];
+      }
+    }
+    if (ch_base < nchan) contract(db, nkb, bn);
     if (pre) {
       s_x[tid] = pq.x;
       s_y[tid] = pq.y;
       s_z[tid] = pq.z;
       s_key[tid] = pkey;
     }
-  };
-
-  // ---- rings ----------------------------------------------------------------
-  auto planned = [&](uint32_t rr) { return J.plan != nullptr && (rr - r_begin) < kPlanRings; };
-  auto plan_lo = [&](uint32_t rr) { return L.plan[2 + 2 * (rr - r_begin)]; };
-  auto plan_hi = [&](uint32_t rr) { return L.plan[3 + 2 * (rr - r_begin)]; };
-  // the first ring at or after rr with samples, or past the plan
-  auto next_ring = [&](uint32_t rr) {
-    while (rr <= r_end && planned(rr) && plan_lo(rr) >= plan_hi(rr)) ++rr;
-    return rr;
-  };
-
-  uint32_t buf = 0;     // alternates per published range: a buffer is rewritten only after a later barrier
-  uint32_t staged = 0;  // records of the current chunk already in LDS (from the previous contraction)
-  for (uint32_t r = (r_begin <= r_end) ? next_ring(r_begin) : 1u; r <= r_end && r_begin <= r_end;
-       r = next_ring(r + 1)) {
-    const bool pl = planned(r);
-    uint32_t lo_u, hi_u;
-    if (pl) {
-      lo_u = plan_lo(r);
-      hi_u = plan_hi(r);
-    } else {
-      // past the plan: wave 0 finds the union's range itself and publishes
-      // it with the cell windows (nothing is staged ahead for such a ring)
-      const uint32_t kr = r - m.firstring;
-      buf ^= 1u;
-      if (wave == 0) {
-        uint32_t lo_r = 0, hi_r = 0;
-        const uint32_t start = B.lut[kr], end = B.lut[kr + 1];
-        if (start != end) {  // uniform
-          uint32_t a, b;
-          ring_cell_window(P, B, m, L.win[lane], r, a, b);
-          L.lp[lane] = a;
-          L.rp[lane] = b;
-          ring_union_range(B, a, b, start, end, lo_r, hi_r);
-        }
-        if (lane == 0) {
-          L.range[buf][0] = lo_r;
-          L.range[buf][1] = hi_r;
-        }
-      }
-      __syncthreads();
-      lo_u = L.range[buf][0];
-      hi_u = L.range[buf][1];
-      if (lo_u >= hi_u) continue;  // uniform
-    }
-    uint32_t lp = 1u, rp = 0u;
-    for (uint32_t c0 = lo_u & ~31u; c0 < hi_u; c0 += CHUNK) {
-      const uint32_t cnt = min(CHUNK, hi_u - c0), nkb = (cnt + 31) / 32;
-      const bool first = c0 == (lo_u & ~31u);
-      __syncthreads();  // the previous chunk's weight tiles and the previous ring's windows are consumed
-      stage(c0, staged, cnt);
-      clear(nkb);
-      // a planned ring's cell windows: wave 0 computes them while the
-      // staging loads are in flight; they go out with the staging barrier
-      if (pl && first && wave == 0) {
-        uint32_t a, b;
-        ring_cell_window(P, B, m, L.win[lane], r, a, b);
-        L.lp[lane] = a;
-        L.rp[lane] = b;
-      }
-      __syncthreads();
-      if (first) {
-        lp = L.lp[c];
-        rp = L.rp[c];
-      }
-      // the next chunk, staged during this contraction when its ring is planned
-      uint32_t nc0 = c0 + CHUNK, ncnt = 0;
-      if (nc0 < hi_u) {
-        ncnt = min(CHUNK, hi_u - nc0);
-      } else {
-        const uint32_t rn = next_ring(r + 1);
-        if (rn <= r_end && planned(rn)) {
-          nc0 = plan_lo(rn) & ~31u;
-          ncnt = min(CHUNK, plan_hi(rn) - nc0);
-        }
-      }
-      staged = min(ncnt, nthreads);
-      weigh_and_contract(c0, cnt, nkb, lp, rp, nc0, staged);
-    }
+    db ^= 1u;
+    __syncthreads();  // T
   }
 
   // per-cell weight sums over the waves (exact in FP64 in any order: every
@@ -754,10 +790,12 @@ extern "C" int hcg_grid_mfma_ex(const hcg_healpix* hp, const hcg_kernel* kern, u
   // with the weights
   uint32_t nw = sh.waves;
   if (nw == 0) {
-    // default: up to 4 waves (chunk 256), two workgroups per CU; measured
-    // faster than one 8-wave workgroup on the 2M x 1024 raster (16.9 vs 18.0 ms)
+    // default: the fewest waves that cover the channels, at most 8 (chunk
+    // 512, one workgroup per CU): with several rings packed per chunk the
+    // 8-wave shape is the fastest on the 2M x 1024 raster (13.5 ms, against
+    // 15.5 for 4 waves at chunk 256, two workgroups per CU)
     nw = 1;
-    while (nw < groups_total && nw < 4u) nw <<= 1;
+    while (nw < groups_total && nw < 8u) nw <<= 1;
   }
   const uint32_t ngroups = ceil_div_u32(groups_total, nw);
   const uint32_t tc = sh.tile_cols ? sh.tile_cols : kMcDefaultTileCols;

@@ -221,7 +221,7 @@ int hcg_grid_mfma(const hcg_healpix *hp, const hcg_kernel *kern, uint32_t side, 
  * 64), with `waves` waves (1, 2, 4 or 8; 64 channels each; more channels
  * take more workgroups, each recomputing the block's weights) and stages
  * `chunk` samples at a time (128, 256 or 512).  0 in a field = the default
- * (tile_cols 16; the fewest waves that cover the channels, at most 4; chunk
+ * (tile_cols 16; the fewest waves that cover the channels, at most 8; chunk
  * by waves).  No counterpart in the reference: the shapes an autotuner
  * picks per workload. */
 typedef struct hcg_mfma_shape {
