@@ -204,7 +204,7 @@ struct MfmaLds {
   __attribute__((aligned(16))) uint16_t wl[kMcCols * (CHUNK + 8)];
   double x[CHUNK], y[CHUNK], z[CHUNK];
   uint32_t key[CHUNK];
-  uint32_t mb[4][2];
+  uint32_t mb[2][4][2];  // row blocks' k-block spans, by chunk parity
   double wsum[kMcCols];
   double p[kMcCols][3];          // cell unit vectors
   uint32_t span[2];              // the block's ring span
@@ -400,7 +400,7 @@ __device__ __forceinline__ void mfma_block(const MfmaJob& J, uint32_t tile, uint
       }
     }
   };
-  // zero the weight tiles' used width and the row blocks' k-block spans
+  // zero the weight tiles' used width
   auto clear = [&](uint32_t nkb) {
     const uint32_t width8 = nkb * 4;  // 16-byte groups per row
     for (uint32_t g = tid; g < kMcCols * width8; g += nthreads) {
@@ -408,9 +408,13 @@ __device__ __forceinline__ void mfma_block(const MfmaJob& J, uint32_t tile, uint
       *reinterpret_cast<uint4*>(&s_wh[rr * WS + cc]) = uint4{0u, 0u, 0u, 0u};
       *reinterpret_cast<uint4*>(&s_wl[rr * WS + cc]) = uint4{0u, 0u, 0u, 0u};
     }
+  };
+  // reset the row blocks' k-block spans of buffer db (read last by the
+  // contraction two chunks back, written by the bands of the next chunk)
+  auto reset_spans = [&](uint32_t db) {
     if (tid < 4) {
-      s_mb[tid][0] = 0xFFFFFFFFu;
-      s_mb[tid][1] = 0u;
+      s_mb[db][tid][0] = 0xFFFFFFFFu;
+      s_mb[db][tid][1] = 0u;
     }
   };
   // each cell's band in each segment of the chunk (wave w: segments w, w +
@@ -425,8 +429,8 @@ __device__ __forceinline__ void mfma_block(const MfmaJob& J, uint32_t tile, uint
         ja = lower_bound_u32(s_key, p0, p1, a);
         jb = upper_bound_u32(s_key, ja, p1, b);
         if (ja < jb) {
-          atomicMin(&s_mb[lane >> 4][0], ja >> 5);
-          atomicMax(&s_mb[lane >> 4][1], (jb + 31) >> 5);
+          atomicMin(&s_mb[db][lane >> 4][0], ja >> 5);
+          atomicMax(&s_mb[db][lane >> 4][1], (jb + 31) >> 5);
         }
       }
       L.band[sg][lane] = ja | (jb << 16);
@@ -464,8 +468,8 @@ __device__ __forceinline__ void mfma_block(const MfmaJob& J, uint32_t tile, uint
     uint32_t mblo[4], mbhi[4];
 #pragma unroll
     for (int mb = 0; mb < 4; ++mb) {
-      mblo[mb] = s_mb[mb][0];
-      mbhi[mb] = s_mb[mb][1];
+      mblo[mb] = s_mb[db][mb][0];
+      mbhi[mb] = s_mb[db][mb][1];
     }
     // software pipeline: block u's MFMAs run while block u+1's channel values load
     for (uint32_t u = 0; u < nkb; ++u) {
@@ -490,19 +494,25 @@ __device__ __forceinline__ void mfma_block(const MfmaJob& J, uint32_t tile, uint
   };
 
   // ---- chunks ---------------------------------------------------------------
-  // barriers per chunk: staging (B1), bands (B2), weights (B3), contraction
-  // done (T).  The next chunk's descriptor is built by wave 0 between B2 and
-  // B3; its first nthreads records load during the contraction and go to
-  // LDS after it (the contraction reads only weight tiles and channel values).
+  // barriers per chunk: staging (B1, only when records are left to stage),
+  // bands (B2), weights (B3), contraction done (T).  The next chunk's
+  // descriptor is built by wave 0 between B2 and B3; its first nthreads
+  // records load during the contraction and go to LDS after it (the
+  // contraction reads only weight tiles and channel values), so a chunk of
+  // at most nthreads samples needs no staging of its own.
   uint32_t db = 0, staged = 0;
   if (wave == 0) build_desc(0);
+  reset_spans(0);
   __syncthreads();
   for (;;) {
     const uint32_t nkb = L.dn[db][0], nseg = L.dn[db][1];
     if (nkb == 0) break;  // uniform
-    stage(db, staged, nkb);
     clear(nkb);
-    __syncthreads();  // B1
+    reset_spans(db ^ 1u);
+    if (staged < nkb * 32u) {  // uniform
+      stage(db, staged, nkb);
+      __syncthreads();  // B1
+    }
     bands(db, nseg);
     __syncthreads();  // B2
     // channel values of the chunk's first k-block, in flight while the weights are computed
@@ -547,21 +557,33 @@ __device__ __forceinline__ void mfma_block(const MfmaJob& J, uint32_t tile, uint
     if (group == 0 && wsum_out && live) wsum_out[cell] = (float)t;
   }
   __syncthreads();
+  // lane l holds cells 16 mb + 4 (l / 16) + j, j = 0..3: four consecutive
+  // cells of one map row (tile widths are multiples of 4), stored as one
+  // 16-byte piece when the map side is a multiple of 4 (then aligned)
   if (ch_base < nchan) {
+    const bool vec4 = (P.side & 3u) == 0;
 #pragma unroll
     for (int mb = 0; mb < 4; ++mb) {
+      const uint32_t cl0 = mb * 16 + (lane >> 4) * 4;
+      const uint32_t rowj = y0 + (cl0 >> tc_shift), col0 = x0 + (cl0 & tc_mask);
+      if (col0 >= P.side || rowj >= P.side) continue;
 #pragma unroll
       for (int nb = 0; nb < kMcNB; ++nb) {
         const uint32_t ch = ch_base + nb * 16 + (lane & 15);
         if (ch >= nchan) continue;
+        float o[4];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-          const uint32_t cl = mb * 16 + (lane >> 4) * 4 + j;
-          const uint32_t rowj = y0 + (cl >> tc_shift), colj = x0 + (cl & tc_mask);
-          if (colj >= P.side || rowj >= P.side) continue;
           const double v = (double)acc[mb][nb][j];
-          const float o = normalize ? (float)(v / s_wsum[cl]) : (float)v;
-          cube[(size_t)ch * M + (size_t)rowj * P.side + colj] = o;
+          o[j] = normalize ? (float)(v / s_wsum[cl0 + j]) : (float)v;
+        }
+        float* dst = cube + (size_t)ch * M + (size_t)rowj * P.side + col0;
+        if (vec4) {
+          *reinterpret_cast<float4*>(dst) = make_float4(o[0], o[1], o[2], o[3]);
+        } else {
+#pragma unroll
+          for (int j = 0; j < 4; ++j)
+            if (col0 + j < P.side) dst[j] = o[j];
         }
       }
     }
