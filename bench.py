@@ -29,6 +29,9 @@ sys.path.insert(0, ROOT)
 
 METRIC = "sample·channel updates/s and p50/p99 job latency at max rel err ≤1e-3 vs FP64"
 HBM_PEAK_GBS = 8000.0
+# VALU issue capacity: 256 CUs x 4 SIMDs, one wave64 VALU instruction per
+# SIMD every 4 cycles, at the 2.4 GHz peak engine clock
+SIMDS, VALU_CYCLES, CLOCK_HZ = 1024, 4, 2.4e9
 FP64_VECTOR_PEAK_TFLOPS = 78.6
 
 
@@ -67,6 +70,21 @@ def measured_traffic(key: str):
         return int(v) if v is not None else None
     except (OSError, ValueError):
         return None
+
+
+def valu_issue(key: str, avg_ms: float):
+    """VALU issue share of a launch: the wave instructions rocprofv3 counted
+    for it (SQ_INSTS_VALU, kept in profiles/traffic.json under valu_insts)
+    at 4 cycles each over 1024 SIMDs, against its measured time."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            insts = json.load(f).get("valu_insts", {}).get(key)
+    except (OSError, ValueError):
+        insts = None
+    if insts is None or not avg_ms:
+        return None
+    issue_ms = insts * VALU_CYCLES / (SIMDS * CLOCK_HZ) * 1e3
+    return {"insts": insts, "issue_ms": issue_ms, "frac": issue_ms / avg_ms}
 
 
 def grid_algorithmic_bytes(n: int, m: int) -> int:
@@ -248,6 +266,7 @@ def bench_cube(args, log):
             "traffic": measured_traffic(f"k_grid_mfma/{wl.name}"),
             "algorithmic_bytes": abytes,
             "avg_ms": ms_grid,
+            "valu": valu_issue(f"k_grid_mfma/{wl.name}", ms_grid),
         },
     }
 
@@ -515,7 +534,9 @@ def main():
             "traffic": measured_traffic(f"k_grid_tiles/{wl.name}"),
             "algorithmic_bytes": abytes,
             "avg_ms": grid_avg,
-            "note": "FP64-VALU-bound kernel: the HBM fraction is low by construction",
+            "valu": valu_issue(f"k_grid_tiles/{wl.name}", grid_avg),
+            "note": "FP64-VALU-bound kernel: the HBM fraction is low by construction; valu.frac is its share "
+                    "of the SIMDs' VALU issue slots",
         },
     }
 
