@@ -220,8 +220,9 @@ struct MfmaLds {
 };
 
 // CHUNK = staged samples per chunk.  LDS: two weight tiles of 64 x (CHUNK+8)
-// elements (16-byte row padding) + CHUNK unit vectors and keys: 148 KB at
-// 512 (one 8-wave workgroup per CU), 75 KB at 256, 38 KB at 128.
+// elements (16-byte row padding) + CHUNK unit vectors and keys + the chunk
+// descriptors and bands: 157 KB at 512 (one workgroup per CU), 80 KB at 256
+// (two per CU), 47 KB at 128.
 //
 // The 64 cells of a workgroup form a TR x TC block of the map (TC = 1 <<
 // tc_shift columns, TR = 64 / TC rows); cell c of the block is (c / TC,
