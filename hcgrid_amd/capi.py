@@ -24,6 +24,8 @@ MAX_Y = 1073741824
 GAUSS1D, GAUSS2D, TAPERED_SINC = 1, 2, 3
 SELECT_COMPAT, SELECT_DISC = 0, 1
 SELECTIONS = {"compat": SELECT_COMPAT, "disc": SELECT_DISC}
+# zenithal target projections (HCG_PROJ_*)
+PROJECTIONS = {"SIN": 0, "TAN": 1, "ARC": 2, "ZEA": 3, "STG": 4}
 DTYPE_F64, DTYPE_F32, DTYPE_BF16, DTYPE_F16 = 0, 1, 2, 3
 
 
@@ -131,6 +133,8 @@ PROTOTYPES = {
                                     C.POINTER(MfmaShape), _vp]),
     "hcg_normalize": (_int, [_u64, _vp, _vp, _vp, _vp]),
     "hcg_eval_exp_neg": (_int, [_u32, _vp, _vp, _vp]),
+    "hcg_wcs_p2s": (_int, [_u32, _u32, _u32, _dbl, _dbl, _dbl, _dbl, _dbl, _dbl, _dbl, _vp, _vp]),
+    "hcg_wcs_p2s_device": (_int, [_u32, _u32, _u32, _dbl, _dbl, _dbl, _dbl, _dbl, _dbl, _dbl, _vp, _vp, _vp]),
     "hcg_wcs_sin_p2s": (_int, [_u32, _u32, _dbl, _dbl, _dbl, _dbl, _dbl, _dbl, _dbl, _vp, _vp]),
     "hcg_wcs_sin_p2s_device": (_int, [_u32, _u32, _dbl, _dbl, _dbl, _dbl, _dbl, _dbl, _dbl, _vp, _vp, _vp]),
     "hcg_geometry_bytes": (_sz, [_u32, _u32, C.POINTER(Healpix)]),

@@ -171,25 +171,64 @@ extern "C" void hcg_host_ring_info(const hcg_healpix* hp, uint64_t ring, uint64_
 
 extern "C" double hcg_host_pix2theta(const hcg_healpix* hp, uint64_t pix) { return hp ? pix2theta(*hp, pix) : 0.0; }
 
-// SIN deprojection (Calabretta & Greisen 2002, Paper II sec. 5.1.5, PV2_1 =
-// PV2_2 = 0) and the native -> celestial rotation (Paper II eq. 2) with
-// (alpha_p, delta_p) = CRVAL and phi_p = LONPOLE.  Cell (x, y) <- pixel
-// (x+1, y+1) as set_WCS builds pixcrd (HCGrid/gmap.cpp:209-238).
+// Zenithal deprojections (Calabretta & Greisen 2002, Paper II sec. 5.1:
+// SIN with PV2_1 = PV2_2 = 0, TAN, ARC, ZEA, STG) and the native ->
+// celestial rotation (Paper II eq. 2) with (alpha_p, delta_p) = CRVAL and
+// phi_p = LONPOLE.  Cell (x, y) <- pixel (x+1, y+1) as set_WCS builds pixcrd
+// (HCGrid/gmap.cpp:209-238).
 namespace {
 
-// One SIN-projection cell (Calabretta & Greisen 2002): intermediate world
-// coordinates (x, y) in radians -> (ra, dec) in degrees; NaN outside the
-// projection's boundary.
-__host__ __device__ inline void wcs_sin_cell(double x, double y, double a0, double sd0, double cd0, double php,
-                                             double& ra_deg, double& dec_deg) {
+// One zenithal-projection cell: intermediate world coordinates (x, y) in
+// radians -> native longitude phi = atan2(x, -y) and native latitude theta
+// from R = sqrt(x^2 + y^2) through the projection's R(theta) (radians):
+//   SIN R = cos(theta)            TAN R = cot(theta)
+//   ARC R = pi/2 - theta          ZEA R = 2 sin((pi/2 - theta) / 2)
+//   STG R = 2 tan((pi/2 - theta) / 2)
+// -> (ra, dec) in degrees; NaN outside the projection's domain (SIN R > 1,
+// ARC R > pi, ZEA R > 2).
+__host__ __device__ inline void wcs_zenithal_cell(uint32_t proj, double x, double y, double a0, double sd0,
+                                                  double cd0, double php, double& ra_deg, double& dec_deg) {
   const double r2 = x * x + y * y;
-  if (r2 > 1.0) {
+  double sth, cth;  // sin and cos of the native latitude
+  bool ok = true;
+  switch (proj) {
+    case HCG_PROJ_TAN: {
+      const double q = 1.0 / sqrt(1.0 + r2);
+      sth = q;
+      cth = sqrt(r2) * q;
+      break;
+    }
+    case HCG_PROJ_ARC: {
+      const double r = sqrt(r2);
+      ok = r <= kPi;
+      sth = cos(r);
+      cth = sin(r);
+      break;
+    }
+    case HCG_PROJ_ZEA: {
+      ok = r2 <= 4.0;
+      sth = 1.0 - 0.5 * r2;
+      cth = sqrt(r2) * sqrt(ok ? 1.0 - 0.25 * r2 : 0.0);
+      break;
+    }
+    case HCG_PROJ_STG: {
+      const double q = 1.0 / (1.0 + 0.25 * r2);
+      sth = (1.0 - 0.25 * r2) * q;
+      cth = sqrt(r2) * q;
+      break;
+    }
+    default: {  // HCG_PROJ_SIN
+      ok = r2 <= 1.0;
+      cth = sqrt(r2);
+      sth = sqrt(ok ? 1.0 - r2 : 0.0);
+      break;
+    }
+  }
+  if (!ok) {
     ra_deg = NAN;
     dec_deg = NAN;
     return;
   }
-  // native latitude: cos(theta) = r, sin(theta) = sqrt(1 - r^2)
-  const double cth = sqrt(r2), sth = sqrt(1.0 - r2);
   const double phi = atan2(x, -y);
   const double dphi = phi - php;
   const double sdp = sin(dphi), cdp = cos(dphi);
@@ -202,23 +241,25 @@ __host__ __device__ inline void wcs_sin_cell(double x, double y, double a0, doub
   dec_deg = dec * kRad2Deg;
 }
 
-__global__ void __launch_bounds__(256) k_wcs_sin(uint32_t naxis1, uint32_t naxis2, double crpix1, double crpix2,
-                                                 double cdelt1, double cdelt2, double a0, double sd0, double cd0,
-                                                 double php, double* __restrict__ xwcs, double* __restrict__ ywcs) {
+__global__ void __launch_bounds__(256) k_wcs_zenithal(uint32_t proj, uint32_t naxis1, uint32_t naxis2, double crpix1,
+                                                      double crpix2, double cdelt1, double cdelt2, double a0,
+                                                      double sd0, double cd0, double php, double* __restrict__ xwcs,
+                                                      double* __restrict__ ywcs) {
   const uint64_t c = (uint64_t)blockIdx.x * 256 + threadIdx.x;
   if (c >= (uint64_t)naxis1 * naxis2) return;
   const uint32_t yy = (uint32_t)(c / naxis1), xx = (uint32_t)(c % naxis1);
   const double y = cdelt2 * ((double)(yy + 1) - crpix2) * kDeg2Rad;
   const double x = cdelt1 * ((double)(xx + 1) - crpix1) * kDeg2Rad;
-  wcs_sin_cell(x, y, a0, sd0, cd0, php, xwcs[c], ywcs[c]);
+  wcs_zenithal_cell(proj, x, y, a0, sd0, cd0, php, xwcs[c], ywcs[c]);
 }
 
 }  // namespace
 
-extern "C" int hcg_wcs_sin_p2s(uint32_t naxis1, uint32_t naxis2, double crpix1, double crpix2, double cdelt1,
-                               double cdelt2, double crval1, double crval2, double lonpole, double* xwcs,
-                               double* ywcs) {
-  HCG_REQUIRE(xwcs && ywcs, "hcg_wcs_sin_p2s: NULL output");
+extern "C" int hcg_wcs_p2s(uint32_t proj, uint32_t naxis1, uint32_t naxis2, double crpix1, double crpix2,
+                           double cdelt1, double cdelt2, double crval1, double crval2, double lonpole, double* xwcs,
+                           double* ywcs) {
+  HCG_REQUIRE(xwcs && ywcs, "hcg_wcs_p2s: NULL output");
+  HCG_REQUIRE(proj <= HCG_PROJ_STG, "hcg_wcs_p2s: unknown projection");
   const double a0 = crval1 * kDeg2Rad, d0 = crval2 * kDeg2Rad, php = lonpole * kDeg2Rad;
   const double sd0 = sin(d0), cd0 = cos(d0);
   for (uint32_t yy = 0; yy < naxis2; ++yy) {
@@ -226,23 +267,38 @@ extern "C" int hcg_wcs_sin_p2s(uint32_t naxis1, uint32_t naxis2, double crpix1, 
     for (uint32_t xx = 0; xx < naxis1; ++xx) {
       const double x = cdelt1 * ((double)(xx + 1) - crpix1) * kDeg2Rad;
       const uint64_t c = (uint64_t)yy * naxis1 + xx;
-      wcs_sin_cell(x, y, a0, sd0, cd0, php, xwcs[c], ywcs[c]);
+      wcs_zenithal_cell(proj, x, y, a0, sd0, cd0, php, xwcs[c], ywcs[c]);
     }
   }
   return HCG_OK;
 }
 
+extern "C" int hcg_wcs_p2s_device(uint32_t proj, uint32_t naxis1, uint32_t naxis2, double crpix1, double crpix2,
+                                  double cdelt1, double cdelt2, double crval1, double crval2, double lonpole,
+                                  double* xwcs, double* ywcs, void* stream) {
+  HCG_REQUIRE(xwcs && ywcs, "hcg_wcs_p2s_device: NULL output");
+  HCG_REQUIRE(proj <= HCG_PROJ_STG, "hcg_wcs_p2s_device: unknown projection");
+  const uint64_t m = (uint64_t)naxis1 * naxis2;
+  if (m == 0) return HCG_OK;
+  HCG_REQUIRE(m < (1ull << 40), "hcg_wcs_p2s_device: too many cells");
+  const double a0 = crval1 * kDeg2Rad, d0 = crval2 * kDeg2Rad, php = lonpole * kDeg2Rad;
+  const double sd0 = sin(d0), cd0 = cos(d0);
+  k_wcs_zenithal<<<(uint32_t)((m + 255) / 256), 256, 0, as_stream(stream)>>>(
+      proj, naxis1, naxis2, crpix1, crpix2, cdelt1, cdelt2, a0, sd0, cd0, php, xwcs, ywcs);
+  HCG_CHECK_LAUNCH("k_wcs_zenithal");
+  return HCG_OK;
+}
+
+extern "C" int hcg_wcs_sin_p2s(uint32_t naxis1, uint32_t naxis2, double crpix1, double crpix2, double cdelt1,
+                               double cdelt2, double crval1, double crval2, double lonpole, double* xwcs,
+                               double* ywcs) {
+  return hcg_wcs_p2s(HCG_PROJ_SIN, naxis1, naxis2, crpix1, crpix2, cdelt1, cdelt2, crval1, crval2, lonpole, xwcs,
+                     ywcs);
+}
+
 extern "C" int hcg_wcs_sin_p2s_device(uint32_t naxis1, uint32_t naxis2, double crpix1, double crpix2,
                                       double cdelt1, double cdelt2, double crval1, double crval2, double lonpole,
                                       double* xwcs, double* ywcs, void* stream) {
-  HCG_REQUIRE(xwcs && ywcs, "hcg_wcs_sin_p2s_device: NULL output");
-  const uint64_t m = (uint64_t)naxis1 * naxis2;
-  if (m == 0) return HCG_OK;
-  HCG_REQUIRE(m < (1ull << 40), "hcg_wcs_sin_p2s_device: too many cells");
-  const double a0 = crval1 * kDeg2Rad, d0 = crval2 * kDeg2Rad, php = lonpole * kDeg2Rad;
-  const double sd0 = sin(d0), cd0 = cos(d0);
-  k_wcs_sin<<<(uint32_t)((m + 255) / 256), 256, 0, as_stream(stream)>>>(naxis1, naxis2, crpix1, crpix2, cdelt1, cdelt2,
-                                                                      a0, sd0, cd0, php, xwcs, ywcs);
-  HCG_CHECK_LAUNCH("k_wcs_sin");
-  return HCG_OK;
+  return hcg_wcs_p2s_device(HCG_PROJ_SIN, naxis1, naxis2, crpix1, crpix2, cdelt1, cdelt2, crval1, crval2, lonpole,
+                            xwcs, ywcs, stream);
 }

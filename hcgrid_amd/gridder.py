@@ -41,15 +41,28 @@ def prepare_grid_kernel(kernel_type: int, kernel_params, sphere_radius: float, h
     return kern, hp
 
 
-def wcs_sin(side_x: int, side_y: int, crpix1: float, crpix2: float, cdelt1: float, cdelt2: float,
-            crval1: float, crval2: float, lonpole: float = 180.0):
-    """Target cell coordinates (set_WCS, HCGrid/gmap.cpp:183-250), host arrays."""
+def wcs(proj: str, side_x: int, side_y: int, crpix1: float, crpix2: float, cdelt1: float, cdelt2: float,
+        crval1: float, crval2: float, lonpole: float | None = None):
+    """Target cell coordinates of a zenithal projection ("SIN", "TAN", "ARC",
+    "ZEA", "STG"; set_WCS, HCGrid/gmap.cpp:183-250), host arrays.  lonpole
+    defaults to 0 when crval2 >= 90 and to 180 otherwise (Calabretta &
+    Greisen 2002, Paper II sec. 2.4)."""
+    if proj not in capi.PROJECTIONS:
+        raise ValueError(f"projection must be one of {sorted(capi.PROJECTIONS)}")
+    if lonpole is None:
+        lonpole = 0.0 if crval2 >= 90.0 else 180.0
     capi.load()
     xw = np.empty(side_x * side_y)
     yw = np.empty(side_x * side_y)
-    call("hcg_wcs_sin_p2s", side_x, side_y, crpix1, crpix2, cdelt1, cdelt2, crval1, crval2, lonpole,
-         xw.ctypes.data_as(C.c_void_p), yw.ctypes.data_as(C.c_void_p))
+    call("hcg_wcs_p2s", capi.PROJECTIONS[proj], side_x, side_y, crpix1, crpix2, cdelt1, cdelt2, crval1, crval2,
+         lonpole, xw.ctypes.data_as(C.c_void_p), yw.ctypes.data_as(C.c_void_p))
     return xw, yw
+
+
+def wcs_sin(side_x: int, side_y: int, crpix1: float, crpix2: float, cdelt1: float, cdelt2: float,
+            crval1: float, crval2: float, lonpole: float = 180.0):
+    """SIN target cell coordinates (set_WCS, HCGrid/gmap.cpp:183-250), host arrays."""
+    return wcs("SIN", side_x, side_y, crpix1, crpix2, cdelt1, cdelt2, crval1, crval2, lonpole)
 
 
 def _ptr(t) -> C.c_void_p | None:

@@ -270,16 +270,25 @@ int set_WCS() {
     fail("set_WCS: target header lacks CTYPE/CRPIX/CDELT/CRVAL for axes 1 and 2");
     return 0;
   }
-  hcgio::card_double(cards, "LONPOLE", lonpole);
-  const bool lon_first = ct1.size() >= 8 && ct2.size() >= 8 && ct1.substr(4) == "-SIN" && ct2.substr(4) == "-SIN" &&
+  // zenithal projections, native pole at CRVAL: the default LONPOLE is 0
+  // when CRVAL2 reaches the pole and 180 otherwise (Calabretta & Greisen
+  // 2002, Paper II sec. 2.4, theta_0 = 90)
+  if (!hcgio::card_double(cards, "LONPOLE", lonpole)) lonpole = crval2 >= 90.0 ? 0.0 : 180.0;
+  static const char* const kProj[] = {"-SIN", "-TAN", "-ARC", "-ZEA", "-STG"};
+  uint32_t proj = 5;
+  for (uint32_t k = 0; k < 5; ++k)
+    if (ct1.size() >= 8 && ct2.size() >= 8 && ct1.substr(4) == kProj[k] && ct2.substr(4) == kProj[k]) proj = k;
+  const bool lon_first = proj < 5 &&
                          (ct1.substr(0, 4) == "RA--" || ct1.substr(0, 4) == "GLON" || ct1.substr(0, 4) == "ELON") &&
                          (ct2.substr(0, 4) == "DEC-" || ct2.substr(0, 4) == "GLAT" || ct2.substr(0, 4) == "ELAT");
-  double pc12 = 0, pc21 = 0, cd = 0;
+  double pc12 = 0, pc21 = 0, cd = 0, pv21 = 0, pv22 = 0;
   const bool has_rot = hcgio::card_double(cards, "PC1_2", pc12) | hcgio::card_double(cards, "PC2_1", pc21) |
                        hcgio::card_double(cards, "CD1_1", cd);
-  if (!lon_first || (has_rot && (pc12 != 0 || pc21 != 0 || cd != 0))) {
-    fail("set_WCS: only SIN targets with longitude on axis 1 and a diagonal CDELT are supported (got " + ct1 +
-         ", " + ct2 + ")");
+  // SIN's slant parameters (Paper II eq. 38); the other four take none
+  const bool has_pv = hcgio::card_double(cards, "PV2_1", pv21) | hcgio::card_double(cards, "PV2_2", pv22);
+  if (!lon_first || (has_rot && (pc12 != 0 || pc21 != 0 || cd != 0)) || (has_pv && (pv21 != 0 || pv22 != 0))) {
+    fail("set_WCS: only zenithal targets (SIN, TAN, ARC, ZEA, STG) with longitude on axis 1, a diagonal CDELT "
+         "and no PV2 parameters are supported (got " + ct1 + ", " + ct2 + ")");
     return 0;
   }
   const uint32_t y = h_zyx[1], x = h_zyx[2];
@@ -287,7 +296,7 @@ int set_WCS() {
   free(h_ywcs);
   h_xwcs = (double*)malloc(sizeof(double) * (size_t)x * y);
   h_ywcs = (double*)malloc(sizeof(double) * (size_t)x * y);
-  if (!check_abi(hcg_wcs_sin_p2s(x, y, crpix1, crpix2, cdelt1, cdelt2, crval1, crval2, lonpole, h_xwcs, h_ywcs),
+  if (!check_abi(hcg_wcs_p2s(proj, x, y, crpix1, crpix2, cdelt1, cdelt2, crval1, crval2, lonpole, h_xwcs, h_ywcs),
                  "set_WCS"))
     return 0;
   return 1;

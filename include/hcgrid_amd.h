@@ -266,17 +266,36 @@ int hcg_normalize(uint64_t ncells, const double *datacube, const double *weights
 int hcg_eval_exp_neg(uint32_t n, const double *x, double *out, void *stream);
 
 /* ---- host-side target geometry ---- */
-/* set_WCS for a SIN (orthographic) target (HCGrid/gmap.cpp:183-250): cell
- * (x, y) at pixel (x+1, y+1) -> world (lon, lat) degrees, cell index
- * y*naxis1 + x.  Host pointers.  Closed form of Calabretta & Greisen 2002. */
-int hcg_wcs_sin_p2s(uint32_t naxis1, uint32_t naxis2, double crpix1, double crpix2,
-                    double cdelt1, double cdelt2, double crval1, double crval2,
-                    double lonpole, double *xwcs, double *ywcs);
+/* Zenithal projections of the target map (Calabretta & Greisen 2002, Paper
+ * II sec. 5.1).  The reference's set_WCS accepts whatever wcslib parses from
+ * the target header (HCGrid/gmap.cpp:183-250); these cover the zenithal
+ * family with the native pole at CRVAL. */
+#define HCG_PROJ_SIN 0u  /* orthographic, PV2_1 = PV2_2 = 0 */
+#define HCG_PROJ_TAN 1u  /* gnomonic */
+#define HCG_PROJ_ARC 2u  /* zenithal equidistant */
+#define HCG_PROJ_ZEA 3u  /* zenithal equal-area */
+#define HCG_PROJ_STG 4u  /* stereographic */
+
+/* set_WCS for a zenithal target (HCGrid/gmap.cpp:183-250): cell (x, y) at
+ * pixel (x+1, y+1) -> world (lon, lat) degrees, cell index y*naxis1 + x,
+ * NaN outside the projection's domain.  Host pointers.  proj is one of
+ * HCG_PROJ_*. */
+int hcg_wcs_p2s(uint32_t proj, uint32_t naxis1, uint32_t naxis2, double crpix1, double crpix2,
+                double cdelt1, double cdelt2, double crval1, double crval2,
+                double lonpole, double *xwcs, double *ywcs);
 
 /* The same projection evaluated on the device (device outputs, enqueued on
  * stream).  Same formula and operation order; the device sin/cos/atan2/asin
  * are the HIP math library's, so results may differ from the host form in
  * the last bit. */
+int hcg_wcs_p2s_device(uint32_t proj, uint32_t naxis1, uint32_t naxis2, double crpix1, double crpix2,
+                       double cdelt1, double cdelt2, double crval1, double crval2,
+                       double lonpole, double *xwcs, double *ywcs, void *stream);
+
+/* hcg_wcs_p2s / hcg_wcs_p2s_device with proj = HCG_PROJ_SIN. */
+int hcg_wcs_sin_p2s(uint32_t naxis1, uint32_t naxis2, double crpix1, double crpix2,
+                    double cdelt1, double cdelt2, double crval1, double crval2,
+                    double lonpole, double *xwcs, double *ywcs);
 int hcg_wcs_sin_p2s_device(uint32_t naxis1, uint32_t naxis2, double crpix1, double crpix2,
                            double cdelt1, double cdelt2, double crval1, double crval2,
                            double lonpole, double *xwcs, double *ywcs, void *stream);

@@ -107,6 +107,8 @@ def lib():
         L.orc_kernel_weight.restype = C.c_double
         L.orc_kernel_weight.argtypes = [P(OrcGrid), C.c_double, C.c_double]
         L.orc_wcs_sin.restype = None
+        L.orc_wcs_zenithal.restype = None
+        L.orc_wcs_zenithal.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32] + [C.c_double] * 7 + [vp, vp]
         L.orc_wcs_sin.argtypes = [C.c_uint32, C.c_uint32] + [C.c_double] * 6 + [vp, vp]
         _lib = L
     return _lib
@@ -311,6 +313,17 @@ def grid_disc_textbook(lon, lat, data, xwcs, ywcs, sphere_radius, p0, nthreads=0
     lib().orc_grid_disc_textbook(xw.size, lon.size, _p(lon), _p(lat), _p(data), _p(xw), _p(yw),
                                  sphere_radius, p0, _p(dc), _p(wc), nthreads)
     return dc, wc
+
+
+WCS_PROJ = {"SIN": 0, "TAN": 1, "ARC": 2, "ZEA": 3, "STG": 4}
+
+
+def wcs_zenithal(proj, naxis1, naxis2, crpix1, crpix2, cdelt1, cdelt2, crval1, crval2, lonpole=180.0):
+    xw = np.empty(naxis1 * naxis2)
+    yw = np.empty(naxis1 * naxis2)
+    lib().orc_wcs_zenithal(WCS_PROJ[proj], naxis1, naxis2, crpix1, crpix2, cdelt1, cdelt2, crval1, crval2,
+                           lonpole, _p(xw), _p(yw))
+    return xw, yw
 
 
 def wcs_sin(naxis1, naxis2, crpix1, crpix2, cdelt1, cdelt2, crval1, crval2):

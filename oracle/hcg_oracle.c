@@ -682,30 +682,48 @@ void orc_normalize(int64_t ncells, const double *datacube, const double *weights
 /* SIN deprojection + zenithal rotation in long double (Calabretta & Greisen
  * 2002, eqs. 2, 54-55 with PV2_1 = PV2_2 = 0, and the native-to-celestial
  * rotation, eq. 2 of the same paper, with phi_p = LONPOLE = 180 deg). */
-void orc_wcs_sin(uint32_t naxis1, uint32_t naxis2, double crpix1, double crpix2,
-                 double cdelt1, double cdelt2, double crval1, double crval2,
-                 double *xwcs, double *ywcs) {
+void orc_wcs_zenithal(uint32_t proj, uint32_t naxis1, uint32_t naxis2, double crpix1, double crpix2,
+                      double cdelt1, double cdelt2, double crval1, double crval2, double lonpole,
+                      double *xwcs, double *ywcs) {
     const long double pi = 3.14159265358979323846264338327950288L;
     const long double d2r = pi / 180.0L;
-    long double a0 = crval1 * d2r, dl0 = crval2 * d2r;
+    long double a0 = crval1 * d2r, dl0 = crval2 * d2r, php = lonpole * d2r;
     for (uint32_t yy = 0; yy < naxis2; ++yy) {
         for (uint32_t xx = 0; xx < naxis1; ++xx) {
             long double x = (long double)cdelt1 * ((long double)(xx + 1) - crpix1) * d2r;
             long double y = (long double)cdelt2 * ((long double)(yy + 1) - crpix2) * d2r;
             long double r = sqrtl(x * x + y * y);
             long double phi = atan2l(x, -y);        /* native longitude */
-            long double theta = acosl(r);          /* native latitude */
-            long double dphi = phi - pi;           /* phi - phi_p */
+            long double theta;                      /* native latitude, inverse of R(theta) */
+            switch (proj) {
+                case 1: theta = atan2l(1.0L, r); break;                     /* TAN: R = cot(theta) */
+                case 2: theta = r <= pi ? pi / 2 - r : NAN; break;          /* ARC: R = pi/2 - theta */
+                case 3: theta = r <= 2.0L ? pi / 2 - 2 * asinl(r / 2) : NAN; break;  /* ZEA */
+                case 4: theta = pi / 2 - 2 * atanl(r / 2); break;           /* STG */
+                default: theta = acosl(r); break;                           /* SIN: R = cos(theta) */
+            }
+            long double dphi = phi - php;
             long double sd = sinl(theta) * sinl(dl0) + cosl(theta) * cosl(dl0) * cosl(dphi);
             long double dec = asinl(sd);
             long double ra = a0 + atan2l(-cosl(theta) * sinl(dphi),
                                          sinl(theta) * cosl(dl0) - cosl(theta) * sinl(dl0) * cosl(dphi));
             long double ra_deg = ra / d2r;
+            uint64_t c = (uint64_t)yy * naxis1 + xx;
+            if (isnan(theta)) {
+                xwcs[c] = NAN;
+                ywcs[c] = NAN;
+                continue;
+            }
             while (ra_deg < 0) ra_deg += 360.0L;
             while (ra_deg >= 360.0L) ra_deg -= 360.0L;
-            uint64_t c = (uint64_t)yy * naxis1 + xx;
             xwcs[c] = (double)ra_deg;
             ywcs[c] = (double)(dec / d2r);
         }
     }
+}
+
+void orc_wcs_sin(uint32_t naxis1, uint32_t naxis2, double crpix1, double crpix2,
+                 double cdelt1, double cdelt2, double crval1, double crval2,
+                 double *xwcs, double *ywcs) {
+    orc_wcs_zenithal(0, naxis1, naxis2, crpix1, crpix2, cdelt1, cdelt2, crval1, crval2, 180.0, xwcs, ywcs);
 }

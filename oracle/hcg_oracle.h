@@ -146,9 +146,17 @@ double orc_kernel_weight(const orc_grid *g, double distance, double bearing);
 /* output normalisation (HCGrid/gmap.cpp:305-312): map = data / weights */
 void orc_normalize(int64_t ncells, const double *datacube, const double *weightscube, double *map);
 
-/* SIN (orthographic) deprojection, Calabretta & Greisen 2002 sec. 5.1.5 with
- * the zenithal rotation, LONPOLE 180.  Pixel (x+1, y+1) of a NAXIS1 x NAXIS2
- * grid -> (lon, lat) degrees, cell index y*naxis1 + x (HCGrid/gmap.cpp:209-238). */
+/* Zenithal deprojections, Calabretta & Greisen 2002 sec. 5.1, with the
+ * native pole at CRVAL and phi_p = lonpole (degrees).  proj: 0 SIN
+ * (PV2_1 = PV2_2 = 0), 1 TAN, 2 ARC, 3 ZEA, 4 STG.  The native latitude is
+ * taken through the inverse of each projection's R(theta) in long double.
+ * Pixel (x+1, y+1) of a NAXIS1 x NAXIS2 grid -> (lon, lat) degrees, cell
+ * index y*naxis1 + x (HCGrid/gmap.cpp:209-238); NaN outside the domain. */
+void orc_wcs_zenithal(uint32_t proj, uint32_t naxis1, uint32_t naxis2, double crpix1, double crpix2,
+                      double cdelt1, double cdelt2, double crval1, double crval2, double lonpole,
+                      double *xwcs, double *ywcs);
+
+/* orc_wcs_zenithal for SIN with LONPOLE 180 */
 void orc_wcs_sin(uint32_t naxis1, uint32_t naxis2, double crpix1, double crpix2,
                  double cdelt1, double cdelt2, double crval1, double crval2,
                  double *xwcs, double *ywcs);

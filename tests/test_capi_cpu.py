@@ -102,6 +102,27 @@ def test_wcs_matches_oracle():
         assert dx.max() < 1e-10 and np.abs(yw - yo).max() < 1e-10
 
 
+@pytest.mark.parametrize("proj", sorted(capi.PROJECTIONS))
+def test_wcs_zenithal_matches_oracle(proj):
+    # every projection, a polar target (default LONPOLE 0) and, with a wide
+    # CDELT, cells outside SIN/ARC/ZEA's domains (NaN on both sides)
+    for side, cdelt, crval in ((90, 0.01, (60.0, 30.0)), (64, 0.3, (0.1, -45.0)), (41, 9.3, (30.0, 90.0))):
+        xw, yw = gridder.wcs(proj, side, side, side / 2, side / 2, -cdelt, cdelt, *crval)
+        lonpole = 0.0 if crval[1] >= 90.0 else 180.0
+        xo, yo = O.wcs_zenithal(proj, side, side, side / 2, side / 2, -cdelt, cdelt, *crval, lonpole)
+        assert np.array_equal(np.isnan(xw), np.isnan(xo)) and np.array_equal(np.isnan(yw), np.isnan(yo))
+        ok = ~np.isnan(yo) & (yo < 90.0 - 1e-9)  # the longitude of the pole itself is arbitrary
+        dx = np.abs((xw[ok] - xo[ok] + 180.0) % 360.0 - 180.0)
+        assert dx.max() < 1e-9 and np.abs(yw[ok] - yo[ok]).max() < 1e-10
+    with pytest.raises(ValueError):
+        gridder.wcs("CAR", 4, 4, 2, 2, -1, 1, 0, 0)
+    lib = capi.load()
+    buf = np.empty(4)
+    assert lib.hcg_wcs_p2s(5, 2, 2, 1.0, 1.0, -1.0, 1.0, 0.0, 0.0, 180.0, buf.ctypes.data, buf.ctypes.data) == \
+        capi.HCG_ERR_INVALID
+    assert b"unknown projection" in lib.hcg_last_error_message()
+
+
 def test_argument_errors_do_not_touch_the_device():
     lib = capi.load()
     hp = capi.Healpix()

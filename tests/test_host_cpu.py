@@ -116,15 +116,75 @@ def test_target_header_and_wcs():
     assert np.abs(xw - xo).max() < 1e-10 and np.abs(yw - yo).max() < 1e-10
 
 
-def test_set_wcs_rejects_other_projections(tmp_path):
+def _card(key, value):
+    return f"{key:<8}= {value:>20}".ljust(80).encode()
+
+
+def _target_variant(tmp_path, name, *subs):
+    src = open(os.path.join(GOLDEN, "target1.fits"), "rb").read()
+    for a, b in subs:
+        assert len(a) == len(b) and src.count(a) == 1, a
+        src = src.replace(a, b)
+    p = tmp_path / name
+    p.write_bytes(src)
+    return p
+
+
+def _set_wcs(lib, syms, path):
+    _fn(lib, syms, "read_output_map(char const*)", [C.c_char_p])(str(path).encode())
+    return _fn(lib, syms, "set_WCS()", [], C.c_int)()
+
+
+@pytest.mark.parametrize("proj", ["TAN", "ARC", "ZEA", "STG"])
+def test_set_wcs_zenithal_targets(tmp_path, proj):
+    lib, syms = _host()
+    p = _target_variant(tmp_path, proj + ".fits", (b"RA---SIN", b"RA---" + proj.encode()),
+                        (b"DEC--SIN", b"DEC--" + proj.encode()))
+    assert _set_wcs(lib, syms, p) == 1, lib.hcg_host_error()
+    m = 90 * 90
+    xw = np.ctypeslib.as_array(C.cast(C.c_void_p.in_dll(lib, "h_xwcs").value, C.POINTER(C.c_double)), (m,))
+    yw = np.ctypeslib.as_array(C.cast(C.c_void_p.in_dll(lib, "h_ywcs").value, C.POINTER(C.c_double)), (m,))
+    xo, yo = O.wcs_zenithal(proj, 90, 90, 45.0, 45.0, -0.055555555555556, 0.055555555555556, 60.0, 30.0)
+    assert np.abs(xw - xo).max() < 1e-10 and np.abs(yw - yo).max() < 1e-10
+
+
+def test_set_wcs_default_lonpole_at_the_pole(tmp_path):
+    # no LONPOLE card: 0 for a target centred on the pole, else 180
     lib, syms = _host()
     src = open(os.path.join(GOLDEN, "target1.fits"), "rb").read()
-    bad = src.replace(b"RA---SIN", b"RA---TAN").replace(b"DEC--SIN", b"DEC--TAN")
-    p = tmp_path / "tan.fits"
-    p.write_bytes(bad)
-    _fn(lib, syms, "read_output_map(char const*)", [C.c_char_p])(str(p).encode())
-    assert _fn(lib, syms, "set_WCS()", [], C.c_int)() == 0
-    assert b"only SIN" in lib.hcg_host_error()
+    lp = src[src.index(b"LONPOLE "):src.index(b"LONPOLE ") + 80]
+    crv = src[src.index(b"CRVAL2  "):src.index(b"CRVAL2  ") + 80]
+    p = _target_variant(tmp_path, "pole.fits", (lp, b"COMMENT".ljust(80)), (crv, _card("CRVAL2", "90.0")))
+    assert _set_wcs(lib, syms, p) == 1, lib.hcg_host_error()
+    m = 90 * 90
+    xw = np.ctypeslib.as_array(C.cast(C.c_void_p.in_dll(lib, "h_xwcs").value, C.POINTER(C.c_double)), (m,))
+    yw = np.ctypeslib.as_array(C.cast(C.c_void_p.in_dll(lib, "h_ywcs").value, C.POINTER(C.c_double)), (m,))
+    xo, yo = O.wcs_zenithal("SIN", 90, 90, 45.0, 45.0, -0.055555555555556, 0.055555555555556, 60.0, 90.0, 0.0)
+    off = yo < 90.0 - 1e-9  # the longitude of the pole itself is arbitrary
+    assert off.sum() == m - 1
+    dx = np.abs((xw[off] - xo[off] + 180.0) % 360.0 - 180.0)
+    assert dx.max() < 1e-9 and np.abs(yw - yo).max() < 1e-10
+    p = _target_variant(tmp_path, "nolp.fits", (lp, b"COMMENT".ljust(80)))
+    assert _set_wcs(lib, syms, p) == 1
+    xw = np.ctypeslib.as_array(C.cast(C.c_void_p.in_dll(lib, "h_xwcs").value, C.POINTER(C.c_double)), (m,))
+    yw = np.ctypeslib.as_array(C.cast(C.c_void_p.in_dll(lib, "h_ywcs").value, C.POINTER(C.c_double)), (m,))
+    xo, yo = O.wcs_sin(90, 90, 45.0, 45.0, -0.055555555555556, 0.055555555555556, 60.0, 30.0)
+    assert np.abs(xw - xo).max() < 1e-10 and np.abs(yw - yo).max() < 1e-10
+
+
+def test_set_wcs_rejects_other_projections(tmp_path):
+    lib, syms = _host()
+    p = _target_variant(tmp_path, "car.fits", (b"RA---SIN", b"RA---CAR"), (b"DEC--SIN", b"DEC--CAR"))
+    assert _set_wcs(lib, syms, p) == 0
+    assert b"only zenithal" in lib.hcg_host_error()
+    p = _target_variant(tmp_path, "mixed.fits", (b"DEC--SIN", b"DEC--TAN"))
+    assert _set_wcs(lib, syms, p) == 0
+    # SIN with a slant (PV2_1 != 0) is not the plain orthographic projection
+    src = open(os.path.join(GOLDEN, "target1.fits"), "rb").read()
+    rd = src[src.index(b"RADESYS "):src.index(b"RADESYS ") + 80]
+    p = _target_variant(tmp_path, "slant.fits", (rd, _card("PV2_1", "0.1")))
+    assert _set_wcs(lib, syms, p) == 0
+    assert b"no PV2" in lib.hcg_host_error()
 
 
 def test_write_output_map_roundtrip(tmp_path):

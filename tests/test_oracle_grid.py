@@ -184,6 +184,62 @@ def test_kernel_weight_closed_forms():
     assert math.isclose(L.orc_kernel_weight(C.byref(g3), d, 0.0), expect, rel_tol=1e-13)
 
 
+# angular distance rho from the reference point at intermediate radius R
+# (radians), each projection's R(theta) with theta = 90 deg - rho inverted
+_RHO_OF_R = {"SIN": np.arcsin, "TAN": np.arctan, "ARC": lambda r: r,
+             "ZEA": lambda r: 2 * np.arcsin(r / 2), "STG": lambda r: 2 * np.arctan(r / 2)}
+_R_OF_RHO = {"SIN": np.sin, "TAN": np.tan, "ARC": lambda p: p,
+             "ZEA": lambda p: 2 * np.sin(p / 2), "STG": lambda p: 2 * np.tan(p / 2)}
+
+
+@pytest.mark.parametrize("proj", sorted(_RHO_OF_R))
+def test_wcs_zenithal_closed_forms(proj):
+    side, c, a0, d0 = 81, 0.4, 123.0, 21.0
+    xw, yw = O.wcs_zenithal(proj, side, side, 41.0, 41.0, -c, c, a0, d0)
+    k0 = 40 * side + 40
+    assert abs(xw[k0] - a0) < 1e-12 and abs(yw[k0] - d0) < 1e-12
+    # central column: ra = ra0 and dec = dec0 +- rho(R)
+    X = np.arange(side)
+    col = X * side + 40
+    y = np.radians(c * (X + 1 - 41.0))
+    assert np.allclose(yw[col], d0 + np.sign(y) * np.degrees(_RHO_OF_R[proj](np.abs(y))), atol=1e-11)
+    assert np.allclose(xw[col], a0, atol=1e-11)
+    # forward projection of every cell returns its pixel offsets: the
+    # orthographic direction scaled by R(rho) / sin(rho)
+    a, d = np.radians(xw - a0), np.radians(yw)
+    dd = math.radians(d0)
+    ox = np.cos(d) * np.sin(a)
+    oy = np.sin(d) * math.cos(dd) - np.cos(d) * math.sin(dd) * np.cos(a)
+    rho = np.arctan2(np.hypot(ox, oy), np.sin(d) * math.sin(dd) + np.cos(d) * math.cos(dd) * np.cos(a))
+    s = np.where(rho > 0, _R_OF_RHO[proj](rho) / np.where(rho > 0, np.sin(rho), 1.0), 1.0)
+    px = (np.arange(side * side) % side) + 1
+    py = (np.arange(side * side) // side) + 1
+    assert np.allclose(-np.degrees(s * ox), c * (px - 41.0), atol=1e-10)  # CDELT1 < 0
+    assert np.allclose(np.degrees(s * oy), c * (py - 41.0), atol=1e-10)
+    # SIN is the same function as the SIN-only entry
+    if proj == "SIN":
+        xs, ys = O.wcs_sin(side, side, 41.0, 41.0, -c, c, a0, d0)
+        assert np.array_equal(xs, xw) and np.array_equal(ys, yw)
+
+
+@pytest.mark.parametrize("proj,rmax", [("SIN", 1.0), ("ARC", math.pi), ("ZEA", 2.0), ("TAN", None), ("STG", None)])
+def test_wcs_zenithal_domains(proj, rmax):
+    # NaN exactly where R leaves the projection's domain; the pole target
+    # with LONPOLE 0 keeps ra = ra0 + phi (Paper II eq. 2 at delta_p = 90)
+    side, c = 41, 9.3  # no cell lies on a domain boundary
+    xw, yw = O.wcs_zenithal(proj, side, side, 21.0, 21.0, -c, c, 30.0, 90.0, 0.0)
+    px = (np.arange(side * side) % side) + 1 - 21.0
+    py = (np.arange(side * side) // side) + 1 - 21.0
+    r = np.radians(c * np.hypot(px, py))
+    out = np.zeros_like(r, dtype=bool) if rmax is None else r > rmax
+    assert np.array_equal(np.isnan(yw), out) and np.array_equal(np.isnan(xw), out)
+    ok = ~out & (r > 0)
+    assert np.allclose(yw[ok], 90.0 - np.degrees(_RHO_OF_R[proj](r[ok])), atol=1e-10)
+    phi = np.degrees(np.arctan2(np.radians(-c * px), np.radians(-c * py)))
+    dra = (xw[ok] - (30.0 + phi[ok] + 180.0)) % 360.0
+    assert np.all(np.minimum(dra, 360.0 - dra) < 1e-9)
+
+
 def test_wcs_sin_closed_forms():
     side, c = 101, 0.05
     xw, yw = O.wcs_sin(side, side, 51.0, 51.0, -c, c, 123.0, 41.0)
