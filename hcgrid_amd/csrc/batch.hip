@@ -234,7 +234,7 @@ namespace {
 struct BatchPlan {
   uint32_t waves = 0, chunk = 0;
   std::vector<MfmaItem> items;
-  std::vector<uint32_t> tiles, tcols;
+  std::vector<uint32_t> tiles, tcols, tpi, tstreams;  // tile-path jobs: tiles per item, streams; else 0
   std::vector<size_t> voff, poff;
   size_t bytes = 0;
 };
@@ -259,28 +259,47 @@ int plan_batch(uint32_t njobs, const hcg_batch_job* jobs, uint32_t waves, uint32
   p.chunk = chunk;
   // work items: jobs longest item first, each job's tiles in order, a
   // tile's channel groups adjacent (they share the tile's samples in L2)
-  std::vector<uint32_t> order(njobs), groups(njobs);
+  std::vector<uint32_t> order(njobs), groups(njobs), units(njobs);
   std::vector<double> cost(njobs);
   p.tiles.assign(njobs, 0);
   p.tcols.assign(njobs, 0);
+  p.tpi.assign(njobs, 0);
+  p.tstreams.assign(njobs, 0);
+  const uint32_t tile_tpi = tile_items_per_block(chunk, waves);
   uint64_t nitems = 0;
   for (uint32_t j = 0; j < njobs; ++j) {
     const hcg_batch_job& b = jobs[j];
-    const uint32_t tc = b.tile_cols ? b.tile_cols : kMcDefaultTileCols;
-    const uint32_t side = b.geometry->side;
-    p.tcols[j] = tc;
-    p.tiles[j] = (b.nchan && side) ? ceil_div_u32(side, tc) * ceil_div_u32(side, 64 / tc) : 0;
-    groups[j] = ceil_div_u32(ceil_div_u32(b.nchan, kMcChanPerWave), waves);
-    cost[j] = item_cost(b.geometry, tc, std::min<uint32_t>(b.nchan, waves * kMcChanPerWave));
+    const hcg_geometry* g = b.geometry;
+    const uint32_t side = g->side;
     order[j] = j;
-    nitems += (uint64_t)p.tiles[j] * groups[j];
+    const GridParams P = make_grid_params(&g->hp, &g->kern, side);
+    if (one_channel_tiles(P, g->kern.kernel_type, false, b.nchan)) {
+      // one channel: FP64 tile path, tile_tpi tiles of 64 cells per item (one
+      // per wave), or on small maps one tile per item over its ring streams;
+      // the tile loop weighs a pair roughly 8x faster than the MFMA stage
+      p.tcols[j] = kTileCols;
+      p.tiles[j] = side ? tile_count_of(side) : 0;
+      p.tstreams[j] = one_channel_streams(side);
+      p.tpi[j] = p.tstreams[j] > 1 ? 1u : tile_tpi;
+      groups[j] = 1;
+      units[j] = ceil_div_u32(p.tiles[j], p.tpi[j]);
+      cost[j] = item_cost(g, kTileCols, 1) * p.tpi[j] / 8.0;
+    } else {
+      const uint32_t tc = b.tile_cols ? b.tile_cols : kMcDefaultTileCols;
+      p.tcols[j] = tc;
+      p.tiles[j] = (b.nchan && side) ? ceil_div_u32(side, tc) * ceil_div_u32(side, 64 / tc) : 0;
+      groups[j] = ceil_div_u32(ceil_div_u32(b.nchan, kMcChanPerWave), waves);
+      units[j] = p.tiles[j];
+      cost[j] = item_cost(g, tc, std::min<uint32_t>(b.nchan, waves * kMcChanPerWave));
+    }
+    nitems += (uint64_t)units[j] * groups[j];
   }
   if (nitems >= 0xFFFFFFFFull) return set_error(HCG_ERR_INVALID, "hcg_batch_create: too many work items");
   std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return cost[a] > cost[b]; });
   p.items.clear();
   p.items.reserve(nitems);
   for (uint32_t j : order)
-    for (uint32_t t = 0; t < p.tiles[j]; ++t)
+    for (uint32_t t = 0; t < units[j]; ++t)
       for (uint32_t g = 0; g < groups[j]; ++g) p.items.push_back(MfmaItem{j, t, g, 0});
   // one allocation: job table, items, counters, sorted channel values
   Carver c{nullptr};
@@ -293,7 +312,7 @@ int plan_batch(uint32_t njobs, const hcg_batch_job* jobs, uint32_t waves, uint32
     p.voff[j] = c.off;
     c.off += hcg_batch_job_bytes(jobs[j].geometry, jobs[j].nchan);
     p.poff[j] = c.off;
-    c.off = align_up(c.off + (size_t)p.tiles[j] * kPlanStride * sizeof(uint32_t));
+    if (!p.tpi[j]) c.off = align_up(c.off + (size_t)p.tiles[j] * kPlanStride * sizeof(uint32_t));
   }
   p.bytes = c.off;
   return HCG_OK;
@@ -343,7 +362,9 @@ int batch_init(uint32_t njobs, const hcg_batch_job* jobs, const BatchPlan& p, vo
                     ceil_div_u32(g->side, p.tcols[j]),
                     p.tiles[j],
                     (uint32_t)(b.normalize ? 1 : 0),
-                    reinterpret_cast<uint32_t*>(static_cast<char*>(bt->base) + p.poff[j])};
+                    p.tpi[j] ? nullptr : reinterpret_cast<uint32_t*>(static_cast<char*>(bt->base) + p.poff[j]),
+                    p.tpi[j],
+                    p.tstreams[j]};
   }
   hipError_t e = hipSuccess;
   const size_t jb = njobs * sizeof(MfmaJob), ib = p.items.size() * sizeof(MfmaItem);
@@ -418,8 +439,10 @@ extern "C" int hcg_batch_enqueue(hcg_batch* b, void* stream) {
     int rc = hcg_gather_channels(jb.geometry->n, jb.nchan, jb.geometry->perm, jb.values, jb.dtype, 0,
                                  b->values_sorted[j], b->d_counters + 1 + j, stream);
     if (rc != HCG_OK) return rc;
-    rc = launch_mfma_plan(b->host_jobs[j], s);
-    if (rc != HCG_OK) return rc;
+    if (b->host_jobs[j].plan) {
+      rc = launch_mfma_plan(b->host_jobs[j], s);
+      if (rc != HCG_OK) return rc;
+    }
   }
   if (b->nitems == 0) return HCG_OK;
   return launch_mfma_batch(b->chunk, b->waves, b->blocks, b->d_jobs, b->d_items, b->nitems, b->d_counters, s);

@@ -32,6 +32,8 @@
 // exactly; only the products with channel values run in BF16/FP16.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "gridcore.hpp"
 #include "mfma_job.hpp"
 
@@ -63,14 +65,6 @@ __device__ inline f32x4_t mfma16(bool bf, s16x8_t a, s16x8_t b, f32x4_t c) {
                                                    c, 0, 0, 0);
   return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8_t, a), __builtin_bit_cast(f16x8_t, b), c,
                                                 0, 0, 0);
-}
-
-__device__ inline uint16_t round16(float f, bool bf) {
-  return bf ? __builtin_bit_cast(uint16_t, (__bf16)f) : __builtin_bit_cast(uint16_t, (_Float16)f);
-}
-
-__device__ inline float widen16(uint16_t h, bool bf) {
-  return bf ? (float)__builtin_bit_cast(__bf16, h) : (float)__builtin_bit_cast(_Float16, h);
 }
 
 // One k-block of B for kMcNB channel blocks, 16 bytes per lane per block.
@@ -609,6 +603,77 @@ __global__ void __launch_bounds__(64 * kMcMaxWaves) k_grid_mfma(MfmaJob J) {
 // channel group) from a device queue in list order until the list is
 // exhausted; every workgroup exits when its pull passes nitems.  The list
 // is ordered longest-first by the host, so the tail holds the short items.
+// A work item of a one-channel tile-path job.  One stream per tile
+// (tile_streams 1): tiles item * tpi + w, one per wave w < tpi, each in its
+// own kTileLdsBytes slice of the workgroup's LDS (the host keeps tpi within
+// tile_items_per_block).  Several streams: tile `item`, its streams over
+// the waves (one_channel_tile; the same sums for any number of waves).
+static_assert(sizeof(MfmaLds<128>) >= one_channel_lds_bytes(kOneChannelStreams, kOneChannelStreams),
+              "the smallest workgroup LDS holds a one-channel tile's streams");
+
+template <uint32_t CHUNK>
+__device__ __forceinline__ void tile_item(const MfmaJob& J, uint32_t item, MfmaLds<CHUNK>& L, const double* s_exp) {
+  const uint32_t nw = blockDim.x >> 6, wave = threadIdx.x >> 6, lane = __lane_id();
+  char* lds = reinterpret_cast<char*>(&L);
+  const TileOut O{static_cast<const uint16_t*>(J.vs), J.cube, J.wsum, J.dtype == HCG_DTYPE_BF16 ? 1u : 0u,
+                  J.normalize};
+  const uint32_t streams = J.tile_streams;
+  // this wave's tile and streams (one call site of the tile body, so the
+  // persistent kernel carries one copy of it)
+  uint32_t tile, q0, q1, qstep, active;
+  if (streams > 1) {
+    active = nw < streams ? nw : streams;
+    tile = item;
+    q0 = wave;
+    q1 = wave < active ? streams : 0;
+    qstep = active;
+  } else {
+    const uint32_t tpi = J.tiles_per_item;
+    active = tpi;
+    tile = item * tpi + wave;
+    q0 = 0;
+    q1 = (wave < tpi && (wave + 1) * kTileLdsBytes <= sizeof(MfmaLds<CHUNK>) && tile < J.ntiles) ? 1 : 0;
+    qstep = 1;
+  }
+  double* red = reinterpret_cast<double*>(lds + active * kTileLdsBytes);
+  double d = 0.0, w = 0.0;
+  if (q0 < q1) {
+    double* s_x = reinterpret_cast<double*>(lds + wave * kTileLdsBytes);
+    double* s_y = s_x + kStageCap;
+    double* s_z = s_y + kStageCap;
+    double* s_v = s_z + kStageCap;
+    uint32_t* s_key = reinterpret_cast<uint32_t*>(s_v + kStageCap);
+    for (uint32_t q = q0; q < q1; q += qstep) {
+      grid_tile<HCG_GAUSS1D, false, 2, true>(J.P, J.B, tile, J.tiles_x, s_x, s_y, s_z, s_v, s_key, s_exp, O,
+                                             RingStream{q, streams}, &d, &w);
+      if (streams > 1) {
+        red[(q * 64 + lane) * 2] = d;
+        red[(q * 64 + lane) * 2 + 1] = w;
+      }
+    }
+  }
+  if (streams > 1) {
+    // the streams' sums in stream order, as one_channel_tile adds them
+    __syncthreads();
+    if (wave != 0) return;
+    d = red[lane * 2];
+    w = red[lane * 2 + 1];
+    for (uint32_t q = 1; q < streams; ++q) {
+      d += red[(q * 64 + lane) * 2];
+      w += red[(q * 64 + lane) * 2 + 1];
+    }
+  } else if (q0 >= q1) {
+    return;
+  }
+  tile_write(J.P, tile, J.tiles_x, O, d, w);
+}
+
+uint32_t tile_items_per_block(uint32_t chunk, uint32_t waves) {
+  const size_t lds = chunk == 512 ? sizeof(MfmaLds<512>) : chunk == 256 ? sizeof(MfmaLds<256>) : sizeof(MfmaLds<128>);
+  const uint32_t cap = (uint32_t)(lds / kTileLdsBytes);
+  return std::max(1u, std::min(waves, cap));
+}
+
 template <uint32_t CHUNK>
 __global__ void __launch_bounds__(64 * kMcMaxWaves)
     k_grid_mfma_batch(const MfmaJob* __restrict__ jobs, const MfmaItem* __restrict__ items, uint32_t nitems,
@@ -625,7 +690,10 @@ __global__ void __launch_bounds__(64 * kMcMaxWaves)
     const uint32_t jb = __builtin_amdgcn_readfirstlane(items[it].job);
     const uint32_t tile = __builtin_amdgcn_readfirstlane(items[it].tile);
     const uint32_t group = __builtin_amdgcn_readfirstlane(items[it].group);
-    mfma_block<kDtAny, CHUNK>(jobs[jb], tile, group, L, s_exp);
+    if (__builtin_amdgcn_readfirstlane(jobs[jb].tiles_per_item))
+      tile_item(jobs[jb], tile, L, s_exp);
+    else
+      mfma_block<kDtAny, CHUNK>(jobs[jb], tile, group, L, s_exp);
     __syncthreads();  // LDS and s_item are free for the next item
   }
 }
@@ -806,6 +874,11 @@ extern "C" int hcg_grid_mfma_ex(const hcg_healpix* hp, const hcg_kernel* kern, u
   GridParams P = make_grid_params(hp, kern, side);
   GridBuffers B{records, keys_sorted, lon_sorted, lat_sorted, weights_sorted, lut, ring_z, ring_s, meta,
                 xwcs,    ywcs,        nullptr,    nullptr};
+  if (one_channel_tiles(P, kern->kernel_type, weights_sorted != nullptr, nchan)) {
+    const TileOut O{static_cast<const uint16_t*>(values_sorted), cube, wsum, dtype == HCG_DTYPE_BF16 ? 1u : 0u,
+                    (uint32_t)(normalize ? 1 : 0)};
+    return launch_tiles_one_channel(P, B, O, s);
+  }
   const uint32_t n_pad = hcg_channels_padded_len(n);
   const uint32_t groups_total = ceil_div_u32(nchan, kMcChanPerWave);  // waves needed
   // waves per workgroup: a power of two (each cell's nw threads are aligned
@@ -832,7 +905,7 @@ extern "C" int hcg_grid_mfma_ex(const hcg_healpix* hp, const hcg_kernel* kern, u
   const uint32_t chunk = sh.chunk ? sh.chunk : (nw >= 8 ? 512u : (nw >= 4 ? 256u : 128u));
   const uint32_t ktc = kern->kernel_type == HCG_GAUSS1D && gauss_small_radius(P) ? HCG_GAUSS1D + 16u : kern->kernel_type;
   MfmaJob J{P,        B,       values_sorted, cube,   wsum, ktc, n_pad, nchan, (uint32_t)dtype,
-            tc_shift, tiles_x, ntiles,        (uint32_t)(normalize ? 1 : 0), nullptr};
+            tc_shift, tiles_x, ntiles,        (uint32_t)(normalize ? 1 : 0), nullptr, 0u, 0u};
   if (workspace) {
     if (workspace_bytes < (size_t)ntiles * kPlanStride * sizeof(uint32_t))
       return set_error(HCG_ERR_WORKSPACE, "hcg_grid_mfma_ex: workspace smaller than hcg_grid_mfma_workspace_size");

@@ -45,46 +45,9 @@
 #include "expneg.hpp"
 #include "healpix.hpp"
 
-#ifndef HCG_GRID_TILE_COLS
-#define HCG_GRID_TILE_COLS 16
-#endif
-#ifndef HCG_GRID_CELLS_PER_LANE
-#define HCG_GRID_CELLS_PER_LANE 1
-#endif
-#ifndef HCG_GRID_STAGE_CAP
-#define HCG_GRID_STAGE_CAP 256
-#endif
-#ifndef HCG_GRID_WAVES_PER_EU
-#define HCG_GRID_WAVES_PER_EU 4
-#endif
-#ifndef HCG_GRID_LANE_LOOP
-// 0: the wave walks the union of its lanes' bands (broadcast LDS reads; lanes
-// outside their band only evaluate the distance); 1: each lane walks its own
-// band (per-lane LDS addresses).  Measured on the 20M raster: 6.69 ms vs 7.14.
-#define HCG_GRID_LANE_LOOP 0
-#endif
-#ifndef HCG_GRID_SMALLR
-#define HCG_GRID_SMALLR 1  // GAUSS1D at R < ~1.1 deg: no series/asin switch, no exp clamp in the inner loop
-#endif
-#ifndef HCG_GRID_TABLE_EXP
-#define HCG_GRID_TABLE_EXP 1
-#endif
+#include "plain_tile.hpp"
 
 namespace hcg {
-
-constexpr int kTileCols = HCG_GRID_TILE_COLS;
-constexpr int kLaneRows = 64 / kTileCols;
-constexpr int kCellsPerLane = HCG_GRID_CELLS_PER_LANE;
-constexpr int kTileRows = kLaneRows * kCellsPerLane;
-constexpr uint32_t kStageCap = HCG_GRID_STAGE_CAP;
-static_assert(kTileCols * kLaneRows == 64, "tile columns must divide 64");
-
-}  // namespace hcg
-
-#include "gridcore.hpp"
-
-namespace hcg {
-
 
 // WMODE (GAUSS1D): 0 general exponent and exp; 1 gauss_small_radius; 2 gauss_tiny_radius.
 template <int KT, bool HAS_W, int WMODE>
@@ -94,166 +57,43 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(HCG_GRI
   __shared__ uint32_t s_key[kStageCap];
   __shared__ double s_exp[64];
   exp_table_load(s_exp);  // visible after the first staging barrier
-
-  const hcg_ring_meta m = *B.meta;
-  if (m.usedrings == 0) return;
   // XCD-aware order: workgroups are dealt round-robin over the 8 XCDs, so
   // block b runs on XCD b % 8; give each XCD a contiguous run of tiles so
   // neighbouring tiles (which stage overlapping samples) share its L2.
   const uint32_t ntiles = tiles_x * tiles_y;
   uint32_t tile = blockIdx.x;
   if ((ntiles & 7u) == 0) tile = (blockIdx.x & 7u) * (ntiles >> 3) + (blockIdx.x >> 3);
-  const uint32_t lane = __lane_id();
-  const uint32_t col = (tile % tiles_x) * kTileCols + (lane % kTileCols);
-  const uint32_t row0 = (tile / tiles_x) * kTileRows + (lane / kTileCols) * kCellsPerLane;
+  grid_tile<KT, HAS_W, WMODE, false>(P, B, tile, tiles_x, s_x, s_y, s_z, s_v, s_key, s_exp, TileOut{},
+                                     RingStream{0, 1}, nullptr, nullptr);
+}
 
-  uint32_t cell[kCellsPerLane];
-  double px[kCellsPerLane], py[kCellsPerLane], pz[kCellsPerLane];
-  double acc_d[kCellsPerLane], acc_w[kCellsPerLane];
-  Window win[kCellsPerLane];
-  uint32_t lane_ur = 0xFFFFFFFFu, lane_dr = 0;
-#pragma unroll
-  for (int k = 0; k < kCellsPerLane; ++k) {
-    const uint32_t row = row0 + k;
-    const bool live = (col < P.side) && (row < P.side);
-    const uint32_t rr = live ? row : 0, cc = live ? col : 0;
-    cell[k] = rr * P.side + cc;
-    const uint32_t anchor = rr * P.side + (cc / P.factor) * P.factor;
-    win[k] = window_of(P, m.firstring, m.usedrings, B.xwcs[anchor], B.ywcs[anchor]);
-    if (!live) {  // an empty ring range: never active
-      win[k].uring = 1;
-      win[k].dring = 0;
-    }
-    double sb, cb, sl, cl;
-    sincos(B.ywcs[cell[k]] * kDeg2Rad, &sb, &cb);
-    sincos(B.xwcs[cell[k]] * kDeg2Rad, &sl, &cl);
-    px[k] = cb * cl;
-    py[k] = cb * sl;
-    pz[k] = sb;
-    acc_d[k] = live ? B.dc[cell[k]] : 0.0;
-    acc_w[k] = live ? B.wc[cell[k]] : 0.0;
-    if (win[k].uring <= win[k].dring) {
-      lane_ur = min(lane_ur, win[k].uring);
-      lane_dr = max(lane_dr, win[k].dring);
-    }
-  }
-  uint32_t r_begin = wave_min_u32(lane_ur);
-  const uint32_t r_end = wave_max_u32(lane_dr);
-  if (r_begin == m.firstring && skips_first_ring(P)) ++r_begin;  // the first used ring contributes nothing
+// A one-channel job of the multi-channel entry points on the tile path:
+// 16-bit values in, FP32 cube and weight sum out; one tile per workgroup of
+// S waves, one partial ring stream per wave (one_channel_tile).
+template <uint32_t S>
+__global__ void __launch_bounds__(64 * S) k_grid_tiles_1ch(GridParams P, GridBuffers B, TileOut O, uint32_t tiles_x,
+                                                           uint32_t tiles_y) {
+  __shared__ __attribute__((aligned(16))) char lds[one_channel_lds_bytes(S, S)];
+  __shared__ double s_exp[64];
+  if (threadIdx.x < 64) exp_table_load(s_exp);
+  __syncthreads();
+  const uint32_t ntiles = tiles_x * tiles_y;
+  uint32_t tile = blockIdx.x;
+  if ((ntiles & 7u) == 0) tile = (blockIdx.x & 7u) * (ntiles >> 3) + (blockIdx.x >> 3);
+  one_channel_tile(P, B, O, tile, tiles_x, S, S, lds, s_exp);
+}
 
-  for (uint32_t r = r_begin; r <= r_end && r_begin <= r_end; ++r) {
-    const uint32_t kr = r - m.firstring;
-    const uint32_t start = B.lut[kr], end = B.lut[kr + 1];
-    if (start == end) continue;
-    const double rz = B.ring_z[kr], rs = B.ring_s[kr];
-    uint32_t lp[kCellsPerLane], rp[kCellsPerLane];
-    uint32_t lane_lp = 0xFFFFFFFFu, lane_rp = 0;
-#pragma unroll
-    for (int k = 0; k < kCellsPerLane; ++k) {
-      uint64_t a = 1, b = 0;
-      const bool act = r >= win[k].uring && r <= win[k].dring && ring_window(P, win[k], r, rz, rs, a, b) && a <= b;
-      lp[k] = act ? (uint32_t)a : 1u;  // inactive: the empty window [1, 0]
-      rp[k] = act ? (uint32_t)b : 0u;
-      if (act) {
-        lane_lp = min(lane_lp, lp[k]);
-        lane_rp = max(lane_rp, rp[k]);
-      }
-    }
-    const uint32_t LP = wave_min_u32(lane_lp), RP = wave_max_u32(lane_rp);
-    if (LP > RP) continue;
-    const uint32_t lo_u = wave_search<false>(B.keys, start, end, LP);
-    const uint32_t hi_u = wave_search<true>(B.keys, lo_u, end, RP);
-    for (uint32_t cb = lo_u; cb < hi_u; cb += kStageCap) {
-      const uint32_t cnt = min(kStageCap, hi_u - cb);
-      __syncthreads();  // previous chunk fully consumed
-      for (uint32_t j = lane; j < cnt; j += 64) {
-        const hcg_sample q = B.rec[cb + j];
-        s_x[j] = q.x;
-        s_y[j] = q.y;
-        s_z[j] = q.z;
-        s_v[j] = q.v;
-        s_key[j] = B.keys[cb + j];
-      }
-      __syncthreads();
-      uint32_t lo[kCellsPerLane], len[kCellsPerLane];
-      uint32_t jb = 0xFFFFFFFFu, je = 0;
-#pragma unroll
-      for (int k = 0; k < kCellsPerLane; ++k) {
-        const uint32_t a = lower_bound_u32(s_key, 0, cnt, lp[k]);
-        const uint32_t b = (lp[k] <= rp[k]) ? upper_bound_u32(s_key, a, cnt, rp[k]) : a;
-        lo[k] = a;
-        len[k] = b - a;
-        if (b > a) {
-          jb = min(jb, a);
-          je = max(je, b);
-        }
-      }
-#if HCG_GRID_LANE_LOOP
-      // each lane walks its own band: the trip count is the wave's longest
-      // band, not the bands' union (over half of which a lane would skip);
-      // per cell the samples are still taken in sorted order
-      uint32_t tmax = 0;
-#pragma unroll
-      for (int k = 0; k < kCellsPerLane; ++k) tmax = max(tmax, len[k]);
-      tmax = wave_max_u32(tmax);
-      (void)jb;
-      (void)je;
-      for (uint32_t t = 0; t < tmax; ++t) {
-#pragma unroll
-        for (int k = 0; k < kCellsPerLane; ++k) {
-          if (t >= len[k]) continue;
-          const uint32_t j = lo[k] + t;
-          const double qx = s_x[j], qy = s_y[j], qz = s_z[j], qv = s_v[j];
-          const double dx = px[k] - qx, dy = py[k] - qy, dz = pz[k] - qz;
-          const double c2 = fma(dx, dx, fma(dy, dy, dz * dz));
-          if (c2 < P.c_hi) {
-#elif HCG_GRID_CELLS_PER_LANE == 1
-      // one cell per lane: its band is exactly [lo, lo + len)
-      (void)jb;
-      (void)je;
-      for (uint32_t t = 0; t < len[0]; ++t) {
-        const uint32_t j = lo[0] + t;
-        const double qx = s_x[j], qy = s_y[j], qz = s_z[j], qv = s_v[j];
-        {
-          constexpr int k = 0;
-          const double dx = px[k] - qx, dy = py[k] - qy, dz = pz[k] - qz;
-          const double c2 = fma(dx, dx, fma(dy, dy, dz * dz));
-          if (c2 < P.c_hi) {
-#else
-      for (uint32_t j = jb; j < je; ++j) {
-        const double qx = s_x[j], qy = s_y[j], qz = s_z[j], qv = s_v[j];
-#pragma unroll
-        for (int k = 0; k < kCellsPerLane; ++k) {
-          const double dx = px[k] - qx, dy = py[k] - qy, dz = pz[k] - qz;
-          const double c2 = fma(dx, dx, fma(dy, dy, dz * dz));
-          if ((j - lo[k]) < len[k] && c2 < P.c_hi) {
-#endif
-            if (c2 < P.c_lo || radius_test_ref(B.xwcs, B.ywcs, B.lon_s, B.lat_s, cell[k], cb + j, P.sphere_radius)) {
-              double wt;
-              if constexpr (KT == HCG_GAUSS1D && WMODE > 0)
-                wt = gauss_weight_series<WMODE>(P, c2, s_exp);
-              else if constexpr (KT == HCG_GAUSS1D)
-                wt = HCG_GRID_TABLE_EXP ? exp_scaled_neg(gauss_exponent_scaled(P, c2), s_exp)
-                                        : exp(gauss_exponent_scaled(P, c2) * 0x1.62e42fefa39efp-7);
-              else
-                wt = weight_general<KT>(P.params[0], P.params[1], P.params[2], B.xwcs, B.ywcs, B.lon_s, B.lat_s,
-                                        0.25 * c2, cell[k], cb + j);
-              if constexpr (HAS_W) wt = B.w_s[cb + j] * wt;
-              acc_d[k] = fma(qv, wt, acc_d[k]);
-              acc_w[k] += wt;
-            }
-          }
-        }
-      }
-    }
-  }
-#pragma unroll
-  for (int k = 0; k < kCellsPerLane; ++k) {
-    if (col < P.side && row0 + k < P.side) {
-      B.dc[cell[k]] = acc_d[k];
-      B.wc[cell[k]] = acc_w[k];
-    }
-  }
+int launch_tiles_one_channel(const GridParams& P, const GridBuffers& B, const TileOut& O, hipStream_t s) {
+  const uint32_t tiles_x = tile_cols_of(P.side);
+  const uint32_t tiles_y = (P.side + kTileRows - 1) / kTileRows;
+  if ((uint64_t)tiles_x * tiles_y == 0) return HCG_OK;
+  if (one_channel_streams(P.side) == kOneChannelStreams)
+    k_grid_tiles_1ch<kOneChannelStreams><<<tiles_x * tiles_y, 64 * kOneChannelStreams, 0, s>>>(P, B, O, tiles_x,
+                                                                                              tiles_y);
+  else
+    k_grid_tiles_1ch<1><<<tiles_x * tiles_y, 64, 0, s>>>(P, B, O, tiles_x, tiles_y);
+  HCG_CHECK_LAUNCH("k_grid_tiles_1ch");
+  return HCG_OK;
 }
 
 // ---------------------------------------------------------------------------

@@ -212,6 +212,15 @@ __device__ inline double gauss_exponent_series(const GridParams& P, double c2) {
 
 // True when every selected pair (c2 < c_hi) takes the series branch and its
 // scaled exponent stays above exp_scaled_neg's clamp (host side).
+// 16-bit values (BF16 or FP16) to and from FP32, round to nearest even.
+__device__ inline uint16_t round16(float f, bool bf) {
+  return bf ? __builtin_bit_cast(uint16_t, (__bf16)f) : __builtin_bit_cast(uint16_t, (_Float16)f);
+}
+
+__device__ inline float widen16(uint16_t h, bool bf) {
+  return bf ? (float)__builtin_bit_cast(__bf16, h) : (float)__builtin_bit_cast(_Float16, h);
+}
+
 inline bool gauss_small_radius(const GridParams& P) {
   if (!(P.c_hi < 4e-4)) return false;
   const double y = P.c_hi * (P.k0 + P.c_hi * (P.k1 + P.c_hi * (P.k2 + P.c_hi * P.k3)));

@@ -4,7 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
-#include "gridcore.hpp"
+#include "plain_tile.hpp"
 
 namespace hcg {
 
@@ -32,6 +32,13 @@ struct MfmaJob {
   uint32_t kt, n_pad, nchan, dtype;  // kt: kernel type; HCG_GAUSS1D + 16 = GAUSS1D at a small radius
   uint32_t tc_shift, tiles_x, ntiles, normalize;
   uint32_t* plan;     // [ntiles][kPlanStride], or NULL (search in the gridding kernel)
+  // 0: the MFMA contraction; else a one-channel job on the FP64 tile path
+  // (one_channel_tiles) with tiles_x and ntiles counting kTileCols x
+  // kTileRows tiles: this many tiles per work item, one per wave, when
+  // tile_streams is 1; one tile per item split over tile_streams partial
+  // ring streams otherwise (one_channel_streams)
+  uint32_t tiles_per_item;
+  uint32_t tile_streams;
 };
 
 // One unit of the persistent launch: a tile of one job, one channel group.
@@ -45,6 +52,10 @@ inline uint32_t tile_shift_of(uint32_t tile_cols) {
 
 // Enqueues the tile-plan kernel of a job into J.plan (non-NULL).
 int launch_mfma_plan(const MfmaJob& J, hipStream_t s);
+
+// Tiles per work item of a one-channel tile-path job in the persistent kernel
+// of (chunk, waves): one per wave, as many as the workgroup's LDS holds.
+uint32_t tile_items_per_block(uint32_t chunk, uint32_t waves);
 
 // Grid size of the persistent kernel: CUs x resident workgroups per CU.
 int mfma_batch_blocks(uint32_t chunk, uint32_t waves, uint32_t* blocks);

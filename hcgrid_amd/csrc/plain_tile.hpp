@@ -1,0 +1,367 @@
+// plain_tile.hpp -- one tile of the single-channel gather-convolution, the
+// body of k_grid_tiles (grid_plain.hip), shared with the persistent batch
+// kernel (grid_mfma.hip), which grids one-channel jobs with it.
+//
+// A tile is kTileCols x kTileRows map cells worked by one wave: lane l owns
+// column l % kTileCols and kCellsPerLane vertically adjacent cells.  The
+// selection, distance, weight and accumulation order are the ones described
+// in grid_plain.hip.  Two instantiations:
+//  * BATCH false (k_grid_tiles): FP64 values from the sample records,
+//    sums accumulated in place into the FP64 data and weight cubes (the
+//    reference's datacube/weightscube), workgroup barriers (one wave per
+//    workgroup);
+//  * BATCH true (a one-channel job of the multi-channel entry points): the
+//    job's sorted 16-bit values, one partial stream of the tile's rings
+//    (every stride-th ring from the phase-th), sums from zero returned to
+//    the caller, which adds the streams in stream order and writes the FP32
+//    cube (divided by the weight sum when the job normalises; NaN where no
+//    sample contributes) and FP32 weight sum (tile_write); wave-level
+//    synchronisation only, since the other waves of the workgroup work
+//    other tiles or streams.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "common.hpp"
+#include "expneg.hpp"
+#include "healpix.hpp"
+
+#ifndef HCG_GRID_TILE_COLS
+#define HCG_GRID_TILE_COLS 16
+#endif
+#ifndef HCG_GRID_CELLS_PER_LANE
+#define HCG_GRID_CELLS_PER_LANE 1
+#endif
+#ifndef HCG_GRID_STAGE_CAP
+#define HCG_GRID_STAGE_CAP 256
+#endif
+#ifndef HCG_GRID_WAVES_PER_EU
+#define HCG_GRID_WAVES_PER_EU 4
+#endif
+#ifndef HCG_GRID_LANE_LOOP
+// 0: the wave walks the union of its lanes' bands (broadcast LDS reads; lanes
+// outside their band only evaluate the distance); 1: each lane walks its own
+// band (per-lane LDS addresses).  Measured on the 20M raster: 6.69 ms vs 7.14.
+#define HCG_GRID_LANE_LOOP 0
+#endif
+#ifndef HCG_GRID_SMALLR
+#define HCG_GRID_SMALLR 1  // GAUSS1D at R < ~1.1 deg: no series/asin switch, no exp clamp in the inner loop
+#endif
+#ifndef HCG_GRID_TABLE_EXP
+#define HCG_GRID_TABLE_EXP 1
+#endif
+
+namespace hcg {
+
+constexpr int kTileCols = HCG_GRID_TILE_COLS;
+constexpr int kLaneRows = 64 / kTileCols;
+constexpr int kCellsPerLane = HCG_GRID_CELLS_PER_LANE;
+constexpr int kTileRows = kLaneRows * kCellsPerLane;
+constexpr uint32_t kStageCap = HCG_GRID_STAGE_CAP;
+static_assert(kTileCols * kLaneRows == 64, "tile columns must divide 64");
+
+}  // namespace hcg
+
+#include "gridcore.hpp"
+
+namespace hcg {
+
+// LDS ordering among the lanes of one wave: its LDS instructions execute in
+// issue order, so a compiler fence at wavefront scope is enough.
+template <bool WAVE>
+__device__ inline void tile_sync() {
+  if constexpr (WAVE) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
+// A one-channel job's values and outputs (BATCH true).
+struct TileOut {
+  const uint16_t* vs;  // sorted values, BF16 or FP16
+  float* cube;         // [side * side]
+  float* wsum;         // [side * side] or NULL
+  uint32_t bf16;       // vs holds BF16 (else FP16)
+  uint32_t normalize;  // cube = sums / weight sums
+};
+
+// The part of a tile's rings one call of grid_tile walks: rings r_begin +
+// phase, + phase + stride, ... (the whole range: phase 0, stride 1).
+struct RingStream {
+  uint32_t phase, stride;
+};
+
+// Per-wave LDS of a tile: staged x, y, z, value columns and keys.
+constexpr size_t kTileLdsBytes = (size_t)kStageCap * (4 * sizeof(double) + sizeof(uint32_t));
+
+// Tiles of the kTileCols x kTileRows tiling of a side x side map.
+inline uint32_t tile_cols_of(uint32_t side) { return (side + kTileCols - 1) / kTileCols; }
+inline uint32_t tile_count_of(uint32_t side) { return tile_cols_of(side) * ((side + kTileRows - 1) / kTileRows); }
+
+// One-channel jobs of the multi-channel entry points (hcg_grid_mfma*, batch
+// jobs) that take the FP64 tile path instead of the MFMA contraction:
+// GAUSS1D, no per-sample weights, a radius small enough for the three-term
+// weight series.  With one channel the FP64 weight is all the work, and
+// the tile loop evaluates it several times faster than the MFMA kernel's
+// weight stage; the result is also closer to FP64 (unrounded weights).
+inline bool one_channel_tiles(const GridParams& P, uint32_t kernel_type, bool weighted, uint32_t nchan) {
+  return nchan == 1 && kernel_type == HCG_GAUSS1D && !weighted && gauss_tiny_radius(P);
+}
+
+// Partial streams per tile of a one-channel job, fixed by the map alone (so
+// a job's sums do not depend on the launch shape): maps of fewer than
+// kOneChannelMinTiles tiles (four per SIMD of the 256-CU chip) split each
+// tile's rings over kOneChannelStreams waves, ring r to stream
+// (r - first ring) % streams; larger maps take one wave per tile.  (A
+// 5M-sample 128^2 map, 256 tiles: 25 ms on one wave per tile, against 15
+// ms on the MFMA kernel, which spreads a block's weights over its waves.)
+constexpr uint32_t kOneChannelMinTiles = 4096;
+constexpr uint32_t kOneChannelStreams = 4;
+inline uint32_t one_channel_streams(uint32_t side) {
+  return tile_count_of(side) < kOneChannelMinTiles ? kOneChannelStreams : 1u;
+}
+
+// Launches the tile path for one one-channel job: one workgroup per tile,
+// one_channel_streams(side) waves.
+int launch_tiles_one_channel(const GridParams& P, const GridBuffers& B, const TileOut& O, hipStream_t s);
+
+// WMODE (GAUSS1D): 0 general exponent and exp; 1 gauss_small_radius; 2 gauss_tiny_radius.
+// s_exp: the 64-entry exponential table (exp_table_load), visible to the wave.
+template <int KT, bool HAS_W, int WMODE, bool BATCH>
+__device__ __forceinline__ void grid_tile(const GridParams P, const GridBuffers B, uint32_t tile,
+                                          uint32_t tiles_x, double* __restrict__ s_x, double* __restrict__ s_y,
+                                          double* __restrict__ s_z, double* __restrict__ s_v,
+                                          uint32_t* __restrict__ s_key, const double* __restrict__ s_exp,
+                                          const TileOut& O, RingStream S, double* sum_d, double* sum_w) {
+  static_assert(!BATCH || kCellsPerLane == 1, "the one-channel path keeps one cell per lane");
+  const hcg_ring_meta m = *B.meta;
+  if constexpr (!BATCH) {
+    if (m.usedrings == 0) return;
+  }
+  const uint32_t lane = __lane_id();
+  const uint32_t col = (tile % tiles_x) * kTileCols + (lane % kTileCols);
+  const uint32_t row0 = (tile / tiles_x) * kTileRows + (lane / kTileCols) * kCellsPerLane;
+
+  uint32_t cell[kCellsPerLane];
+  double px[kCellsPerLane], py[kCellsPerLane], pz[kCellsPerLane];
+  double acc_d[kCellsPerLane], acc_w[kCellsPerLane];
+  Window win[kCellsPerLane];
+  uint32_t lane_ur = 0xFFFFFFFFu, lane_dr = 0;
+#pragma unroll
+  for (int k = 0; k < kCellsPerLane; ++k) {
+    const uint32_t row = row0 + k;
+    const bool live = (col < P.side) && (row < P.side);
+    const uint32_t rr = live ? row : 0, cc = live ? col : 0;
+    cell[k] = rr * P.side + cc;
+    const uint32_t anchor = rr * P.side + (cc / P.factor) * P.factor;
+    win[k] = window_of(P, m.firstring, m.usedrings, B.xwcs[anchor], B.ywcs[anchor]);
+    if (!live || (BATCH && m.usedrings == 0)) {  // an empty ring range: never active
+      win[k].uring = 1;
+      win[k].dring = 0;
+    }
+    double sb, cb, sl, cl;
+    sincos(B.ywcs[cell[k]] * kDeg2Rad, &sb, &cb);
+    sincos(B.xwcs[cell[k]] * kDeg2Rad, &sl, &cl);
+    px[k] = cb * cl;
+    py[k] = cb * sl;
+    pz[k] = sb;
+    if constexpr (BATCH) {
+      acc_d[k] = 0.0;
+      acc_w[k] = 0.0;
+    } else {
+      acc_d[k] = live ? B.dc[cell[k]] : 0.0;
+      acc_w[k] = live ? B.wc[cell[k]] : 0.0;
+    }
+    if (win[k].uring <= win[k].dring) {
+      lane_ur = min(lane_ur, win[k].uring);
+      lane_dr = max(lane_dr, win[k].dring);
+    }
+  }
+  uint32_t r_begin = wave_min_u32(lane_ur);
+  const uint32_t r_end = wave_max_u32(lane_dr);
+  if (r_begin == m.firstring && skips_first_ring(P)) ++r_begin;  // the first used ring contributes nothing
+
+  for (uint32_t r = r_begin + S.phase; r <= r_end && r_begin <= r_end; r += S.stride) {
+    const uint32_t kr = r - m.firstring;
+    const uint32_t start = B.lut[kr], end = B.lut[kr + 1];
+    if (start == end) continue;
+    const double rz = B.ring_z[kr], rs = B.ring_s[kr];
+    uint32_t lp[kCellsPerLane], rp[kCellsPerLane];
+    uint32_t lane_lp = 0xFFFFFFFFu, lane_rp = 0;
+#pragma unroll
+    for (int k = 0; k < kCellsPerLane; ++k) {
+      uint64_t a = 1, b = 0;
+      const bool act = r >= win[k].uring && r <= win[k].dring && ring_window(P, win[k], r, rz, rs, a, b) && a <= b;
+      lp[k] = act ? (uint32_t)a : 1u;  // inactive: the empty window [1, 0]
+      rp[k] = act ? (uint32_t)b : 0u;
+      if (act) {
+        lane_lp = min(lane_lp, lp[k]);
+        lane_rp = max(lane_rp, rp[k]);
+      }
+    }
+    const uint32_t LP = wave_min_u32(lane_lp), RP = wave_max_u32(lane_rp);
+    if (LP > RP) continue;
+    const uint32_t lo_u = wave_search<false>(B.keys, start, end, LP);
+    const uint32_t hi_u = wave_search<true>(B.keys, lo_u, end, RP);
+    for (uint32_t cb = lo_u; cb < hi_u; cb += kStageCap) {
+      const uint32_t cnt = min(kStageCap, hi_u - cb);
+      tile_sync<BATCH>();  // previous chunk fully consumed
+      for (uint32_t j = lane; j < cnt; j += 64) {
+        const hcg_sample q = B.rec[cb + j];
+        s_x[j] = q.x;
+        s_y[j] = q.y;
+        s_z[j] = q.z;
+        if constexpr (BATCH)
+          s_v[j] = (double)widen16(O.vs[cb + j], O.bf16 != 0);
+        else
+          s_v[j] = q.v;
+        s_key[j] = B.keys[cb + j];
+      }
+      tile_sync<BATCH>();
+      uint32_t lo[kCellsPerLane], len[kCellsPerLane];
+      uint32_t jb = 0xFFFFFFFFu, je = 0;
+#pragma unroll
+      for (int k = 0; k < kCellsPerLane; ++k) {
+        const uint32_t a = lower_bound_u32(s_key, 0, cnt, lp[k]);
+        const uint32_t b = (lp[k] <= rp[k]) ? upper_bound_u32(s_key, a, cnt, rp[k]) : a;
+        lo[k] = a;
+        len[k] = b - a;
+        if (b > a) {
+          jb = min(jb, a);
+          je = max(je, b);
+        }
+      }
+#if HCG_GRID_LANE_LOOP
+      // each lane walks its own band: the trip count is the wave's longest
+      // band, not the bands' union (over half of which a lane would skip);
+      // per cell the samples are still taken in sorted order
+      uint32_t tmax = 0;
+#pragma unroll
+      for (int k = 0; k < kCellsPerLane; ++k) tmax = max(tmax, len[k]);
+      tmax = wave_max_u32(tmax);
+      (void)jb;
+      (void)je;
+      for (uint32_t t = 0; t < tmax; ++t) {
+#pragma unroll
+        for (int k = 0; k < kCellsPerLane; ++k) {
+          if (t >= len[k]) continue;
+          const uint32_t j = lo[k] + t;
+          const double qx = s_x[j], qy = s_y[j], qz = s_z[j], qv = s_v[j];
+          const double dx = px[k] - qx, dy = py[k] - qy, dz = pz[k] - qz;
+          const double c2 = fma(dx, dx, fma(dy, dy, dz * dz));
+          if (c2 < P.c_hi) {
+#elif HCG_GRID_CELLS_PER_LANE == 1
+      // one cell per lane: its band is exactly [lo, lo + len)
+      (void)jb;
+      (void)je;
+      for (uint32_t t = 0; t < len[0]; ++t) {
+        const uint32_t j = lo[0] + t;
+        const double qx = s_x[j], qy = s_y[j], qz = s_z[j], qv = s_v[j];
+        {
+          constexpr int k = 0;
+          const double dx = px[k] - qx, dy = py[k] - qy, dz = pz[k] - qz;
+          const double c2 = fma(dx, dx, fma(dy, dy, dz * dz));
+          if (c2 < P.c_hi) {
+#else
+      for (uint32_t j = jb; j < je; ++j) {
+        const double qx = s_x[j], qy = s_y[j], qz = s_z[j], qv = s_v[j];
+#pragma unroll
+        for (int k = 0; k < kCellsPerLane; ++k) {
+          const double dx = px[k] - qx, dy = py[k] - qy, dz = pz[k] - qz;
+          const double c2 = fma(dx, dx, fma(dy, dy, dz * dz));
+          if ((j - lo[k]) < len[k] && c2 < P.c_hi) {
+#endif
+            if (c2 < P.c_lo || radius_test_ref(B.xwcs, B.ywcs, B.lon_s, B.lat_s, cell[k], cb + j, P.sphere_radius)) {
+              double wt;
+              if constexpr (KT == HCG_GAUSS1D && WMODE > 0)
+                wt = gauss_weight_series<WMODE>(P, c2, s_exp);
+              else if constexpr (KT == HCG_GAUSS1D)
+                wt = HCG_GRID_TABLE_EXP ? exp_scaled_neg(gauss_exponent_scaled(P, c2), s_exp)
+                                        : exp(gauss_exponent_scaled(P, c2) * 0x1.62e42fefa39efp-7);
+              else
+                wt = weight_general<KT>(P.params[0], P.params[1], P.params[2], B.xwcs, B.ywcs, B.lon_s, B.lat_s,
+                                        0.25 * c2, cell[k], cb + j);
+              if constexpr (HAS_W) wt = B.w_s[cb + j] * wt;
+              acc_d[k] = fma(qv, wt, acc_d[k]);
+              acc_w[k] += wt;
+            }
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < kCellsPerLane; ++k) {
+    if constexpr (BATCH) {
+      sum_d[k] = acc_d[k];
+      sum_w[k] = acc_w[k];
+    } else if (col < P.side && row0 + k < P.side) {
+      B.dc[cell[k]] = acc_d[k];
+      B.wc[cell[k]] = acc_w[k];
+    }
+  }
+}
+
+// The FP32 outputs of one one-channel tile cell (lane) from its sums.
+__device__ inline void tile_write(const GridParams& P, uint32_t tile, uint32_t tiles_x, const TileOut& O, double d,
+                                  double w) {
+  const uint32_t lane = __lane_id();
+  const uint32_t col = (tile % tiles_x) * kTileCols + (lane % kTileCols);
+  const uint32_t row = (tile / tiles_x) * kTileRows + lane / kTileCols;
+  if (col < P.side && row < P.side) {
+    const uint32_t cell = row * P.side + col;
+    O.cube[cell] = O.normalize ? (float)(d / w) : (float)d;
+    if (O.wsum) O.wsum[cell] = (float)w;
+  }
+}
+
+// LDS of one_channel_tile: `active` staging slices, then the streams' sums
+// when there are several.
+constexpr size_t one_channel_lds_bytes(uint32_t active, uint32_t streams) {
+  return active * kTileLdsBytes + (streams > 1 ? (size_t)streams * 64 * 2 * sizeof(double) : 0);
+}
+
+// One one-channel tile over `streams` partial streams: waves w < active of
+// the workgroup walk streams w, w + active, ... in their own kTileLdsBytes
+// slice of `lds`; the streams' sums meet after the slices and wave 0 adds
+// them in stream order and writes the cells.  Every wave of the workgroup
+// calls this (it holds a workgroup barrier when streams > 1).
+__device__ __forceinline__ void one_channel_tile(const GridParams& P, const GridBuffers& B, const TileOut& O,
+                                                 uint32_t tile, uint32_t tiles_x, uint32_t streams, uint32_t active,
+                                                 char* lds, const double* s_exp) {
+  const uint32_t wave = threadIdx.x >> 6, lane = __lane_id();
+  double* red = reinterpret_cast<double*>(lds + active * kTileLdsBytes);
+  double d = 0.0, w = 0.0;
+  if (wave < active) {
+    double* s_x = reinterpret_cast<double*>(lds + wave * kTileLdsBytes);
+    double* s_y = s_x + kStageCap;
+    double* s_z = s_y + kStageCap;
+    double* s_v = s_z + kStageCap;
+    uint32_t* s_key = reinterpret_cast<uint32_t*>(s_v + kStageCap);
+    for (uint32_t q = wave; q < streams; q += active) {
+      grid_tile<HCG_GAUSS1D, false, 2, true>(P, B, tile, tiles_x, s_x, s_y, s_z, s_v, s_key, s_exp, O,
+                                             RingStream{q, streams}, &d, &w);
+      if (streams > 1) {
+        red[(q * 64 + lane) * 2] = d;
+        red[(q * 64 + lane) * 2 + 1] = w;
+      }
+    }
+  }
+  if (streams > 1) {
+    __syncthreads();
+    if (wave == 0) {
+      d = red[lane * 2];
+      w = red[lane * 2 + 1];
+      for (uint32_t q = 1; q < streams; ++q) {
+        d += red[(q * 64 + lane) * 2];
+        w += red[(q * 64 + lane) * 2 + 1];
+      }
+    }
+  }
+  if (wave == 0) tile_write(P, tile, tiles_x, O, d, w);
+}
+
+}  // namespace hcg

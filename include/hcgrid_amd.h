@@ -208,7 +208,13 @@ int hcg_gather_channels(uint32_t n, uint32_t nchan, const uint32_t *perm, const 
  * cube is FP32 [nchan][side*side] (channel-major, like the reference's
  * datacube planes); wsum (FP32 [side*side], may be NULL) receives sum w~.
  * NaN where a cell has no weight.  Samples are the sorted records/keys of
- * hcg_presort/hcg_gather_samples; values_sorted from hcg_gather_channels. */
+ * hcg_presort/hcg_gather_samples; values_sorted from hcg_gather_channels.
+ * One channel (nchan == 1) with GAUSS1D, no weights_sorted, a radius below
+ * ~0.26 deg and side >= 512 (at least 4096 tiles of 16 x 4 cells) runs the
+ * single-channel FP64 tile loop instead of the contraction (w~ = the FP64
+ * weight itself; the launch shape is ignored): with one channel the weight
+ * is all the work, and that loop is several times faster at it.  Batch
+ * jobs take the same route, so a job's cube is the same alone or batched. */
 int hcg_grid_mfma(const hcg_healpix *hp, const hcg_kernel *kern, uint32_t side, uint32_t n, uint32_t nchan,
                   int dtype, const hcg_sample *records, const uint32_t *keys_sorted,
                   const double *lon_sorted, const double *lat_sorted, const double *weights_sorted,

@@ -55,12 +55,24 @@ def _run_case(n_lines, n_per, side, nchan, dtype, layout, seed, dec=30.0, tol=No
     tol = tol or 1e-4
     assert e["max_rel"] <= tol and e["rms_rel"] <= 1e-4
     assert wrel.size > 0 and wrel.max() <= 2.0 ** -14
+    if nchan == 1:  # the FP64 tile path: unrounded weights
+        assert e["max_rel"] <= 1e-6 and wrel.max() <= 1e-6
     return dev, o
 
 
 @pytest.mark.parametrize("nchan", [1, 48, 64, 130])
 def test_cube_bf16_channel_counts(gpu, nchan):
     _run_case(40, 200, 24, nchan, "bf16", 0, seed=nchan)
+
+
+@pytest.mark.parametrize("dtype", ["bf16", "f16"])
+def test_one_channel_tile_route(gpu, dtype):
+    # one channel on a 512^2 map (4096 tiles, one wave each): the FP64 tile
+    # loop instead of the contraction; the samples cover part of the map, the
+    # rest is NaN.  (Smaller maps split each tile over four ring streams:
+    # test_cube_bf16_channel_counts[1].)
+    dev, _ = _run_case(40, 300, 512, 1, dtype, 0, seed=77)
+    assert np.isnan(dev["cube"]).sum() > 0
 
 
 @pytest.mark.parametrize("shape", [(8, 1, 128), (16, 2, 256), (32, 4, 512), (64, 8, 128), (16, 8, 512),
