@@ -265,7 +265,8 @@ int plan_batch(uint32_t njobs, const hcg_batch_job* jobs, uint32_t waves, uint32
   p.tcols.assign(njobs, 0);
   p.tpi.assign(njobs, 0);
   p.tstreams.assign(njobs, 0);
-  const uint32_t tile_tpi = tile_items_per_block(chunk, waves);
+  const uint32_t tile_tpi = tile_items_per_block(chunk, waves, kTileLdsBytes);
+  const uint32_t few_tpi = tile_items_per_block(chunk, waves, kFewTileLdsBytes);
   uint64_t nitems = 0;
   for (uint32_t j = 0; j < njobs; ++j) {
     const hcg_batch_job& b = jobs[j];
@@ -284,6 +285,15 @@ int plan_batch(uint32_t njobs, const hcg_batch_job* jobs, uint32_t waves, uint32
       groups[j] = 1;
       units[j] = ceil_div_u32(p.tiles[j], p.tpi[j]);
       cost[j] = item_cost(g, kTileCols, 1) * p.tpi[j] / 8.0;
+    } else if (few_channel_tiles(P, g->kern.kernel_type, false, b.nchan)) {
+      // few channels on a large map: FP64 tile path, one tile per wave
+      p.tcols[j] = kTileCols;
+      p.tiles[j] = tile_count_of(side);
+      p.tstreams[j] = 1;
+      p.tpi[j] = few_tpi;
+      groups[j] = 1;
+      units[j] = ceil_div_u32(p.tiles[j], few_tpi);
+      cost[j] = item_cost(g, kTileCols, b.nchan) * few_tpi / 4.0;
     } else {
       const uint32_t tc = b.tile_cols ? b.tile_cols : kMcDefaultTileCols;
       p.tcols[j] = tc;

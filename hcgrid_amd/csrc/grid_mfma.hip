@@ -616,7 +616,14 @@ __device__ __forceinline__ void tile_item(const MfmaJob& J, uint32_t item, MfmaL
   const uint32_t nw = blockDim.x >> 6, wave = threadIdx.x >> 6, lane = __lane_id();
   char* lds = reinterpret_cast<char*>(&L);
   const TileOut O{static_cast<const uint16_t*>(J.vs), J.cube, J.wsum, J.dtype == HCG_DTYPE_BF16 ? 1u : 0u,
-                  J.normalize};
+                  J.normalize, J.nchan};
+  if (J.nchan > 1) {  // few-channel tiles, one per wave
+    const uint32_t tpi = J.tiles_per_item;
+    const uint32_t t = item * tpi + wave;
+    if (wave < tpi && (wave + 1) * kFewTileLdsBytes <= sizeof(MfmaLds<CHUNK>) && t < J.ntiles)
+      few_channel_tile(J.P, J.B, O, t, J.tiles_x, lds + wave * kFewTileLdsBytes, s_exp);
+    return;
+  }
   const uint32_t streams = J.tile_streams;
   // this wave's tile and streams (one call site of the tile body, so the
   // persistent kernel carries one copy of it)
@@ -668,9 +675,9 @@ __device__ __forceinline__ void tile_item(const MfmaJob& J, uint32_t item, MfmaL
   tile_write(J.P, tile, J.tiles_x, O, d, w);
 }
 
-uint32_t tile_items_per_block(uint32_t chunk, uint32_t waves) {
+uint32_t tile_items_per_block(uint32_t chunk, uint32_t waves, size_t tile_bytes) {
   const size_t lds = chunk == 512 ? sizeof(MfmaLds<512>) : chunk == 256 ? sizeof(MfmaLds<256>) : sizeof(MfmaLds<128>);
-  const uint32_t cap = (uint32_t)(lds / kTileLdsBytes);
+  const uint32_t cap = (uint32_t)(lds / tile_bytes);
   return std::max(1u, std::min(waves, cap));
 }
 
@@ -876,8 +883,13 @@ extern "C" int hcg_grid_mfma_ex(const hcg_healpix* hp, const hcg_kernel* kern, u
                 xwcs,    ywcs,        nullptr,    nullptr};
   if (one_channel_tiles(P, kern->kernel_type, weights_sorted != nullptr, nchan)) {
     const TileOut O{static_cast<const uint16_t*>(values_sorted), cube, wsum, dtype == HCG_DTYPE_BF16 ? 1u : 0u,
-                    (uint32_t)(normalize ? 1 : 0)};
+                    (uint32_t)(normalize ? 1 : 0), 1u};
     return launch_tiles_one_channel(P, B, O, s);
+  }
+  if (few_channel_tiles(P, kern->kernel_type, weights_sorted != nullptr, nchan)) {
+    const TileOut O{static_cast<const uint16_t*>(values_sorted), cube, wsum, dtype == HCG_DTYPE_BF16 ? 1u : 0u,
+                    (uint32_t)(normalize ? 1 : 0), nchan};
+    return launch_tiles_few_channels(P, B, O, s);
   }
   const uint32_t n_pad = hcg_channels_padded_len(n);
   const uint32_t groups_total = ceil_div_u32(nchan, kMcChanPerWave);  // waves needed

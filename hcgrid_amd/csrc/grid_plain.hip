@@ -83,6 +83,29 @@ __global__ void __launch_bounds__(64 * S) k_grid_tiles_1ch(GridParams P, GridBuf
   one_channel_tile(P, B, O, tile, tiles_x, S, S, lds, s_exp);
 }
 
+// A few-channel job of the multi-channel entry points on the tile path: one
+// wave per tile (few_channel_tile).
+__global__ void __launch_bounds__(64) k_grid_tiles_few(GridParams P, GridBuffers B, TileOut O, uint32_t tiles_x,
+                                                       uint32_t tiles_y) {
+  __shared__ __attribute__((aligned(16))) char lds[kFewTileLdsBytes];
+  __shared__ double s_exp[64];
+  exp_table_load(s_exp);
+  tile_sync<true>();
+  const uint32_t ntiles = tiles_x * tiles_y;
+  uint32_t tile = blockIdx.x;
+  if ((ntiles & 7u) == 0) tile = (blockIdx.x & 7u) * (ntiles >> 3) + (blockIdx.x >> 3);
+  few_channel_tile(P, B, O, tile, tiles_x, lds, s_exp);
+}
+
+int launch_tiles_few_channels(const GridParams& P, const GridBuffers& B, const TileOut& O, hipStream_t s) {
+  const uint32_t tiles_x = tile_cols_of(P.side);
+  const uint32_t tiles_y = (P.side + kTileRows - 1) / kTileRows;
+  if ((uint64_t)tiles_x * tiles_y == 0) return HCG_OK;
+  k_grid_tiles_few<<<tiles_x * tiles_y, 64, 0, s>>>(P, B, O, tiles_x, tiles_y);
+  HCG_CHECK_LAUNCH("k_grid_tiles_few");
+  return HCG_OK;
+}
+
 int launch_tiles_one_channel(const GridParams& P, const GridBuffers& B, const TileOut& O, hipStream_t s) {
   const uint32_t tiles_x = tile_cols_of(P.side);
   const uint32_t tiles_y = (P.side + kTileRows - 1) / kTileRows;

@@ -53,9 +53,10 @@ inline uint32_t tile_shift_of(uint32_t tile_cols) {
 // Enqueues the tile-plan kernel of a job into J.plan (non-NULL).
 int launch_mfma_plan(const MfmaJob& J, hipStream_t s);
 
-// Tiles per work item of a one-channel tile-path job in the persistent kernel
-// of (chunk, waves): one per wave, as many as the workgroup's LDS holds.
-uint32_t tile_items_per_block(uint32_t chunk, uint32_t waves);
+// Tiles per work item of a tile-path job in the persistent kernel of (chunk,
+// waves): one per wave, as many as the workgroup's LDS holds at tile_bytes
+// per tile.
+uint32_t tile_items_per_block(uint32_t chunk, uint32_t waves, size_t tile_bytes);
 
 // Grid size of the persistent kernel: CUs x resident workgroups per CU.
 int mfma_batch_blocks(uint32_t chunk, uint32_t waves, uint32_t* blocks);

@@ -81,11 +81,13 @@ __device__ inline void tile_sync() {
 
 // A one-channel job's values and outputs (BATCH true).
 struct TileOut {
-  const uint16_t* vs;  // sorted values, BF16 or FP16
-  float* cube;         // [side * side]
+  const uint16_t* vs;  // sorted values, BF16 or FP16: [n_pad] (one channel), or 32-sample blocked
+                       // [n_pad / 32][nchan][32] (few channels)
+  float* cube;         // [nchan][side * side]
   float* wsum;         // [side * side] or NULL
   uint32_t bf16;       // vs holds BF16 (else FP16)
   uint32_t normalize;  // cube = sums / weight sums
+  uint32_t nchan;      // channels (few-channel tiles)
 };
 
 // The part of a tile's rings one call of grid_tile walks: rings r_begin +
@@ -124,19 +126,48 @@ inline uint32_t one_channel_streams(uint32_t side) {
   return tile_count_of(side) < kOneChannelMinTiles ? kOneChannelStreams : 1u;
 }
 
+// Few-channel jobs (2..kFewChannels channels, otherwise as one_channel_tiles,
+// on maps of at least kOneChannelMinTiles tiles) also take the tile loop:
+// FP64 weights, FP32 accumulation per channel.  The MFMA kernel pays a
+// fixed chain of barriers and global round trips per 64-cell block, which
+// on sparse large maps outweighs the per-pair work (300k samples x 16
+// channels onto 2048^2: 5.4 ms there, one block per ~40 us per workgroup);
+// the tile loop runs one wave per tile at several waves per SIMD.
+constexpr int kFewChannels = 16;
+constexpr uint32_t kFewCap = 128;  // staged samples per round
+constexpr size_t kFewTileLdsBytes = (size_t)kFewCap * (3 * sizeof(double) + sizeof(uint32_t) + kFewChannels * 2);
+
+inline bool few_channel_tiles(const GridParams& P, uint32_t kernel_type, bool weighted, uint32_t nchan) {
+  return nchan >= 2 && nchan <= (uint32_t)kFewChannels && kernel_type == HCG_GAUSS1D && !weighted &&
+         gauss_tiny_radius(P) && tile_count_of(P.side) >= kOneChannelMinTiles;
+}
+
 // Launches the tile path for one one-channel job: one workgroup per tile,
 // one_channel_streams(side) waves.
 int launch_tiles_one_channel(const GridParams& P, const GridBuffers& B, const TileOut& O, hipStream_t s);
 
+// Launches the tile path for one few-channel job: one wave per tile.
+int launch_tiles_few_channels(const GridParams& P, const GridBuffers& B, const TileOut& O, hipStream_t s);
+
 // WMODE (GAUSS1D): 0 general exponent and exp; 1 gauss_small_radius; 2 gauss_tiny_radius.
 // s_exp: the 64-entry exponential table (exp_table_load), visible to the wave.
-template <int KT, bool HAS_W, int WMODE, bool BATCH>
+//
+// NC > 0 (few-channel tiles, BATCH true): up to NC channels of 16-bit values
+// staged per sample in s_c ([CAP][NC]; s_v unused), one FP32 accumulator per
+// channel and lane, and the cube, weight sum and normalisation written here
+// (no ring streams: S must be {0, 1}).
+template <int KT, bool HAS_W, int WMODE, bool BATCH, int NC = 0, uint32_t CAP = kStageCap>
 __device__ __forceinline__ void grid_tile(const GridParams P, const GridBuffers B, uint32_t tile,
                                           uint32_t tiles_x, double* __restrict__ s_x, double* __restrict__ s_y,
                                           double* __restrict__ s_z, double* __restrict__ s_v,
                                           uint32_t* __restrict__ s_key, const double* __restrict__ s_exp,
-                                          const TileOut& O, RingStream S, double* sum_d, double* sum_w) {
+                                          const TileOut& O, RingStream S, double* sum_d, double* sum_w,
+                                          uint16_t* __restrict__ s_c = nullptr) {
   static_assert(!BATCH || kCellsPerLane == 1, "the one-channel path keeps one cell per lane");
+  static_assert(NC == 0 || (BATCH && NC % 2 == 0), "few-channel tiles: an even channel count, batch outputs");
+  float acc_c[NC > 0 ? NC : 1];
+#pragma unroll
+  for (int ch = 0; ch < (NC > 0 ? NC : 1); ++ch) acc_c[ch] = 0.0f;
   const hcg_ring_meta m = *B.meta;
   if constexpr (!BATCH) {
     if (m.usedrings == 0) return;
@@ -206,19 +237,32 @@ __device__ __forceinline__ void grid_tile(const GridParams P, const GridBuffers 
     if (LP > RP) continue;
     const uint32_t lo_u = wave_search<false>(B.keys, start, end, LP);
     const uint32_t hi_u = wave_search<true>(B.keys, lo_u, end, RP);
-    for (uint32_t cb = lo_u; cb < hi_u; cb += kStageCap) {
-      const uint32_t cnt = min(kStageCap, hi_u - cb);
+    for (uint32_t cb = lo_u; cb < hi_u; cb += CAP) {
+      const uint32_t cnt = min(CAP, hi_u - cb);
       tile_sync<BATCH>();  // previous chunk fully consumed
       for (uint32_t j = lane; j < cnt; j += 64) {
         const hcg_sample q = B.rec[cb + j];
         s_x[j] = q.x;
         s_y[j] = q.y;
         s_z[j] = q.z;
-        if constexpr (BATCH)
+        if constexpr (NC > 0)
+          ;  // channel values below
+        else if constexpr (BATCH)
           s_v[j] = (double)widen16(O.vs[cb + j], O.bf16 != 0);
         else
           s_v[j] = q.v;
         s_key[j] = B.keys[cb + j];
+      }
+      if constexpr (NC > 0) {
+        // channel by channel: lanes read consecutive samples of one channel
+        // (contiguous within a 32-sample block); slots past nchan hold 0
+        for (uint32_t ch = 0; ch < (uint32_t)NC; ++ch) {
+          const bool have = ch < O.nchan;
+          for (uint32_t j = lane; j < cnt; j += 64) {
+            const uint32_t i = cb + j;
+            s_c[j * NC + ch] = have ? O.vs[((size_t)(i >> 5) * O.nchan + ch) * 32 + (i & 31)] : (uint16_t)0;
+          }
+        }
       }
       tile_sync<BATCH>();
       uint32_t lo[kCellsPerLane], len[kCellsPerLane];
@@ -259,7 +303,7 @@ __device__ __forceinline__ void grid_tile(const GridParams P, const GridBuffers 
       (void)je;
       for (uint32_t t = 0; t < len[0]; ++t) {
         const uint32_t j = lo[0] + t;
-        const double qx = s_x[j], qy = s_y[j], qz = s_z[j], qv = s_v[j];
+        const double qx = s_x[j], qy = s_y[j], qz = s_z[j], qv = NC > 0 ? 0.0 : s_v[j];
         {
           constexpr int k = 0;
           const double dx = px[k] - qx, dy = py[k] - qy, dz = pz[k] - qz;
@@ -285,13 +329,51 @@ __device__ __forceinline__ void grid_tile(const GridParams P, const GridBuffers 
                 wt = weight_general<KT>(P.params[0], P.params[1], P.params[2], B.xwcs, B.ywcs, B.lon_s, B.lat_s,
                                         0.25 * c2, cell[k], cb + j);
               if constexpr (HAS_W) wt = B.w_s[cb + j] * wt;
-              acc_d[k] = fma(qv, wt, acc_d[k]);
+              if constexpr (NC > 0) {
+                // two channels per 32-bit LDS word; the value type is the
+                // job's (a uniform branch)
+                const float wf = (float)wt;
+                const uint32_t* cv = reinterpret_cast<const uint32_t*>(s_c + j * NC);
+                if (O.bf16) {
+#pragma unroll
+                  for (int h = 0; h < NC / 2; ++h) {
+                    const uint32_t u = cv[h];
+                    acc_c[2 * h] = fmaf(wf, __uint_as_float(u << 16), acc_c[2 * h]);
+                    acc_c[2 * h + 1] = fmaf(wf, __uint_as_float(u & 0xFFFF0000u), acc_c[2 * h + 1]);
+                  }
+                } else {
+#pragma unroll
+                  for (int h = 0; h < NC / 2; ++h) {
+                    const uint32_t u = cv[h];
+                    acc_c[2 * h] = fmaf(wf, (float)__builtin_bit_cast(_Float16, (uint16_t)(u & 0xFFFFu)),
+                                        acc_c[2 * h]);
+                    acc_c[2 * h + 1] = fmaf(wf, (float)__builtin_bit_cast(_Float16, (uint16_t)(u >> 16)),
+                                            acc_c[2 * h + 1]);
+                  }
+                }
+                (void)qv;
+              } else {
+                acc_d[k] = fma(qv, wt, acc_d[k]);
+              }
               acc_w[k] += wt;
             }
           }
         }
       }
     }
+  }
+  if constexpr (NC > 0) {
+    if (col < P.side && row0 < P.side) {
+      const size_t M = (size_t)P.side * P.side;
+      const uint32_t c0 = cell[0];
+#pragma unroll
+      for (int ch = 0; ch < NC; ++ch)
+        if ((uint32_t)ch < O.nchan)
+          O.cube[(size_t)ch * M + c0] =
+              O.normalize ? (float)((double)acc_c[ch] / acc_w[0]) : acc_c[ch];
+      if (O.wsum) O.wsum[c0] = (float)acc_w[0];
+    }
+    return;
   }
 #pragma unroll
   for (int k = 0; k < kCellsPerLane; ++k) {
@@ -362,6 +444,21 @@ __device__ __forceinline__ void one_channel_tile(const GridParams& P, const Grid
     }
   }
   if (wave == 0) tile_write(P, tile, tiles_x, O, d, w);
+}
+
+// One few-channel tile in a kFewTileLdsBytes slice of LDS (one wave).  Out
+// of line: inlined into the persistent batch kernel as a third copy of the
+// tile body, it raised that kernel's register spills and slowed its MFMA
+// and one-channel items by 1-10%; the call costs a few-channel tile ~4%.
+static __device__ __noinline__ void few_channel_tile(const GridParams& P, const GridBuffers& B, const TileOut& O,
+                                                 uint32_t tile, uint32_t tiles_x, char* lds, const double* s_exp) {
+  double* s_x = reinterpret_cast<double*>(lds);
+  double* s_y = s_x + kFewCap;
+  double* s_z = s_y + kFewCap;
+  uint32_t* s_key = reinterpret_cast<uint32_t*>(s_z + kFewCap);
+  uint16_t* s_c = reinterpret_cast<uint16_t*>(s_key + kFewCap);
+  grid_tile<HCG_GAUSS1D, false, 2, true, kFewChannels, kFewCap>(P, B, tile, tiles_x, s_x, s_y, s_z, nullptr, s_key,
+                                                                s_exp, O, RingStream{0, 1}, nullptr, nullptr, s_c);
 }
 
 }  // namespace hcg

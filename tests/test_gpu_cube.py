@@ -57,12 +57,22 @@ def _run_case(n_lines, n_per, side, nchan, dtype, layout, seed, dec=30.0, tol=No
     assert wrel.size > 0 and wrel.max() <= 2.0 ** -14
     if nchan == 1:  # the FP64 tile path: unrounded weights
         assert e["max_rel"] <= 1e-6 and wrel.max() <= 1e-6
+    elif nchan <= 16 and side >= 512:  # few-channel tiles: FP64 weight sums, FP32 channel sums
+        assert wrel.max() <= 1e-6
     return dev, o
 
 
 @pytest.mark.parametrize("nchan", [1, 48, 64, 130])
 def test_cube_bf16_channel_counts(gpu, nchan):
     _run_case(40, 200, 24, nchan, "bf16", 0, seed=nchan)
+
+
+@pytest.mark.parametrize("nchan,dtype", [(16, "bf16"), (16, "f16"), (6, "bf16")])
+def test_few_channel_tile_route(gpu, nchan, dtype):
+    # 2..16 channels on a 512^2 map: the tile loop with one FP32 sum per
+    # channel (6 channels: the unused slots of the 16 stay out of the cube)
+    dev, _ = _run_case(40, 300, 512, nchan, dtype, 0, seed=78 + nchan)
+    assert np.isnan(dev["cube"]).sum() > 0
 
 
 @pytest.mark.parametrize("dtype", ["bf16", "f16"])
