@@ -1,11 +1,13 @@
 // plain_tile.hpp -- one tile of the single-channel gather-convolution, the
-// body of k_grid_tiles (grid_plain.hip), shared with the persistent batch
-// kernel (grid_mfma.hip), which grids one-channel jobs with it.
+// body of k_grid_tiles (grid_plain.hip), shared with the one- and
+// few-channel jobs of the multi-channel entry points (grid_plain.hip's
+// k_grid_tiles_1ch / k_grid_tiles_few, and the persistent batch kernel in
+// grid_mfma.hip).
 //
 // A tile is kTileCols x kTileRows map cells worked by one wave: lane l owns
 // column l % kTileCols and kCellsPerLane vertically adjacent cells.  The
 // selection, distance, weight and accumulation order are the ones described
-// in grid_plain.hip.  Two instantiations:
+// in grid_plain.hip.  Instantiations:
 //  * BATCH false (k_grid_tiles): FP64 values from the sample records,
 //    sums accumulated in place into the FP64 data and weight cubes (the
 //    reference's datacube/weightscube), workgroup barriers (one wave per
@@ -17,7 +19,9 @@
 //    cube (divided by the weight sum when the job normalises; NaN where no
 //    sample contributes) and FP32 weight sum (tile_write); wave-level
 //    synchronisation only, since the other waves of the workgroup work
-//    other tiles or streams.
+//    other tiles or streams;
+//  * NC > 0 (few-channel jobs): as BATCH true, with one FP32 sum per channel
+//    and the outputs written by the tile itself.
 #pragma once
 
 #include <hip/hip_runtime.h>

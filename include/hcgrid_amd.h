@@ -213,8 +213,11 @@ int hcg_gather_channels(uint32_t n, uint32_t nchan, const uint32_t *perm, const 
  * ~0.26 deg and side >= 512 (at least 4096 tiles of 16 x 4 cells) runs the
  * single-channel FP64 tile loop instead of the contraction (w~ = the FP64
  * weight itself; the launch shape is ignored): with one channel the weight
- * is all the work, and that loop is several times faster at it.  Batch
- * jobs take the same route, so a job's cube is the same alone or batched. */
+ * is all the work, and that loop is several times faster at it.  2..16
+ * channels under the same conditions take that loop too, with one FP32 sum
+ * per channel (weights rounded once to FP32; wsum is the FP64 weight sum):
+ * on sparse maps it avoids the contraction's fixed cost per block.  Batch
+ * jobs take the same routes, so a job's cube is the same alone or batched. */
 int hcg_grid_mfma(const hcg_healpix *hp, const hcg_kernel *kern, uint32_t side, uint32_t n, uint32_t nchan,
                   int dtype, const hcg_sample *records, const uint32_t *keys_sorted,
                   const double *lon_sorted, const double *lat_sorted, const double *weights_sorted,
