@@ -54,10 +54,15 @@ struct hcg_batch {
   MfmaJob* d_jobs;
   MfmaItem* d_items;
   uint32_t* d_counters;  // [0] queue head, [1 + j] non-finite values of job j
+  PrepRange* d_ranges;   // the prep launch: every job's channel gather and tile plan
+  PrepJob* d_prep;
+  uint32_t nranges, prep_blocks;
   std::vector<hcg_batch_job> jobs;
   std::vector<void*> values_sorted;
   std::vector<MfmaJob> host_jobs;     // upload sources, alive until destroy
   std::vector<MfmaItem> host_items;
+  std::vector<PrepRange> host_ranges;
+  std::vector<PrepJob> host_prep;
 };
 
 namespace {
@@ -316,6 +321,8 @@ int plan_batch(uint32_t njobs, const hcg_batch_job* jobs, uint32_t waves, uint32
   c.take<MfmaJob>(njobs ? njobs : 1);
   c.take<MfmaItem>(p.items.empty() ? 1 : p.items.size());
   c.take<uint32_t>(1 + njobs);
+  c.take<PrepRange>(2 * (size_t)njobs + 1);
+  c.take<PrepJob>(njobs ? njobs : 1);
   p.voff.assign(njobs, 0);
   p.poff.assign(njobs, 0);
   for (uint32_t j = 0; j < njobs; ++j) {
@@ -347,6 +354,8 @@ int batch_init(uint32_t njobs, const hcg_batch_job* jobs, const BatchPlan& p, vo
   bt->d_jobs = d.take<MfmaJob>(njobs ? njobs : 1);
   bt->d_items = d.take<MfmaItem>(p.items.empty() ? 1 : p.items.size());
   bt->d_counters = d.take<uint32_t>(1 + njobs);
+  bt->d_ranges = d.take<PrepRange>(2 * (size_t)njobs + 1);
+  bt->d_prep = d.take<PrepJob>(njobs ? njobs : 1);
   bt->values_sorted.resize(njobs);
   std::vector<MfmaJob>& hj = bt->host_jobs;
   hj.resize(njobs);
@@ -376,16 +385,34 @@ int batch_init(uint32_t njobs, const hcg_batch_job* jobs, const BatchPlan& p, vo
                     p.tpi[j],
                     p.tstreams[j]};
   }
+  // the prep launch: gathers (channel-major values) and plans of every job
+  bt->host_prep.resize(njobs);
+  std::vector<uint32_t> layouts(njobs, 0u);
+  for (uint32_t j = 0; j < njobs; ++j)
+    bt->host_prep[j] = PrepJob{jobs[j].values, jobs[j].geometry->perm, bt->d_counters + 1 + j, jobs[j].geometry->n, 0u};
+  int prc = prep_ranges(njobs, hj.data(), layouts.data(), bt->host_ranges, &bt->prep_blocks);
+  if (prc != HCG_OK) {
+    if (owns) (void)hipFree(bt->base);
+    delete bt;
+    return prc;
+  }
+  bt->nranges = (uint32_t)bt->host_ranges.size();
   hipError_t e = hipSuccess;
   const size_t jb = njobs * sizeof(MfmaJob), ib = p.items.size() * sizeof(MfmaItem);
+  const size_t rb = bt->host_ranges.size() * sizeof(PrepRange), pb = njobs * sizeof(PrepJob);
   if (upload_async) {
     // ordered on `stream`; the host sources live in the batch until destroy
     if (jb) e = hipMemcpyAsync(bt->d_jobs, hj.data(), jb, hipMemcpyHostToDevice, stream);
     if (e == hipSuccess && ib)
       e = hipMemcpyAsync(bt->d_items, bt->host_items.data(), ib, hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess && rb)
+      e = hipMemcpyAsync(bt->d_ranges, bt->host_ranges.data(), rb, hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess && pb) e = hipMemcpyAsync(bt->d_prep, bt->host_prep.data(), pb, hipMemcpyHostToDevice, stream);
   } else {
     if (jb) e = hipMemcpy(bt->d_jobs, hj.data(), jb, hipMemcpyHostToDevice);
     if (e == hipSuccess && ib) e = hipMemcpy(bt->d_items, p.items.data(), ib, hipMemcpyHostToDevice);
+    if (e == hipSuccess && rb) e = hipMemcpy(bt->d_ranges, bt->host_ranges.data(), rb, hipMemcpyHostToDevice);
+    if (e == hipSuccess && pb) e = hipMemcpy(bt->d_prep, bt->host_prep.data(), pb, hipMemcpyHostToDevice);
   }
   int rc = HCG_OK;
   if (e != hipSuccess) rc = set_error(HCG_ERR_HIP, std::string("hcg_batch_create: ") + hipGetErrorString(e));
@@ -443,17 +470,9 @@ extern "C" int hcg_batch_enqueue(hcg_batch* b, void* stream) {
   const uint32_t nc = 1 + b->njobs;
   k_fill_u32<<<ceil_div_u32(nc, 256), 256, 0, s>>>(b->d_counters, nc, 0u);
   HCG_CHECK_LAUNCH("k_fill_u32");
-  for (uint32_t j = 0; j < b->njobs; ++j) {
-    const hcg_batch_job& jb = b->jobs[j];
-    if (jb.nchan == 0) continue;
-    int rc = hcg_gather_channels(jb.geometry->n, jb.nchan, jb.geometry->perm, jb.values, jb.dtype, 0,
-                                 b->values_sorted[j], b->d_counters + 1 + j, stream);
-    if (rc != HCG_OK) return rc;
-    if (b->host_jobs[j].plan) {
-      rc = launch_mfma_plan(b->host_jobs[j], s);
-      if (rc != HCG_OK) return rc;
-    }
-  }
+  // every job's channel gather and tile plan, one launch
+  int rc = launch_batch_prep(b->d_ranges, b->nranges, b->prep_blocks, b->d_prep, b->d_jobs, s);
+  if (rc != HCG_OK) return rc;
   if (b->nitems == 0) return HCG_OK;
   return launch_mfma_batch(b->chunk, b->waves, b->blocks, b->d_jobs, b->d_items, b->nitems, b->d_counters, s);
 }

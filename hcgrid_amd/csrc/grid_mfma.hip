@@ -150,9 +150,8 @@ __device__ inline void ring_union_range(const GridBuffers& B, uint32_t lp, uint3
 // The tile plan: one wave per tile, lane = cell.  Many small workgroups per
 // CU hide the latency of the per-ring searches that the gridding kernel,
 // with one or two large workgroups per CU, cannot.
-__global__ void __launch_bounds__(64) k_mfma_plan(MfmaJob J) {
-  const uint32_t tile = blockIdx.x;
-  const uint32_t lane = threadIdx.x;
+__device__ inline void plan_tile(const MfmaJob& J, uint32_t tile) {
+  const uint32_t lane = __lane_id();
   const hcg_ring_meta m = *J.B.meta;
   const uint32_t tc_shift = J.tc_shift;
   const uint32_t y0 = (tile / J.tiles_x) << (6u - tc_shift);
@@ -183,6 +182,8 @@ __global__ void __launch_bounds__(64) k_mfma_plan(MfmaJob J) {
     }
   }
 }
+
+__global__ void __launch_bounds__(64) k_mfma_plan(MfmaJob J) { plan_tile(J, blockIdx.x); }
 
 int launch_mfma_plan(const MfmaJob& J, hipStream_t s) {
   if (J.ntiles == 0) return HCG_OK;
@@ -741,13 +742,70 @@ int launch_mfma_batch(uint32_t chunk, uint32_t waves, uint32_t blocks, const Mfm
 
 // Channel gather into the blocked layout: sorted slot i of channel ch goes
 // to block i / 32, row ch, column i % 32 ([nblk][nchan][32], zero-padded to
-// nblk = hcg_channels_padded_len(n) / 32 blocks).  A thread owns one sorted
-// slot and walks kGatherChans channels, so the permutation is read once per
-// kGatherChans channels rather than once per channel.
+// nblk = hcg_channels_padded_len(n) / 32 blocks).  A workgroup takes
+// kGatherSlots sorted slots x kGatherChans channels through LDS: thread t
+// gathers slot t of each channel (consecutive lanes read the caller's
+// values at consecutive permuted positions, 2 bytes each), and then writes
+// 16-byte pieces of block rows, a block's channel rows being contiguous.
 constexpr uint32_t kGatherChans = 32;
+constexpr uint32_t kGatherSlots = 256;
+constexpr uint32_t kGatherRow = kGatherSlots + 8;  // LDS row stride (16-byte pad against bank conflicts)
 
 __device__ inline size_t blocked_index(uint32_t i, uint32_t ch, uint32_t nchan) {
   return ((size_t)(i >> 5) * nchan + ch) * 32u + (i & 31u);
+}
+
+template <typename T>
+__device__ inline void gather_cm_block(uint32_t bx, uint32_t by, uint32_t n, uint32_t n_pad, uint32_t nchan,
+                                       const uint32_t* __restrict__ perm, const T* __restrict__ in,
+                                       T* __restrict__ out, uint32_t* __restrict__ nonfinite,
+                                       uint16_t* __restrict__ lds /* [kGatherChans][kGatherRow] */) {
+  static_assert(sizeof(T) == 2, "16-bit values");
+  const uint32_t t = threadIdx.x;
+  const uint32_t i0 = bx * kGatherSlots;
+  const uint32_t ch0 = by * kGatherChans;
+  const uint32_t nc = min(kGatherChans, nchan - ch0);
+  const uint32_t i = i0 + t;
+  uint32_t j = 0;
+  if (i < n) {
+    j = perm[i];
+    if (j >= n) j = i;  // a permutation never holds this; keeps every read in bounds regardless
+  }
+  // all of the thread's loads first (up to kGatherChans in flight), then
+  // the LDS stores
+  uint16_t x[kGatherChans];
+  const bool live = i < n;
+  const T* src = in + (size_t)ch0 * n + j;
+#pragma unroll
+  for (uint32_t c = 0; c < kGatherChans; ++c)
+    x[c] = (live && c < nc) ? __builtin_bit_cast(uint16_t, src[(size_t)c * n]) : (uint16_t)0;
+  uint32_t bad = 0;
+#pragma unroll
+  for (uint32_t c = 0; c < kGatherChans; ++c) {
+    if (c < nc) {
+      bad += isfinite((float)__builtin_bit_cast(T, x[c])) ? 0u : 1u;
+      lds[c * kGatherRow + t] = x[c];
+    }
+  }
+  if (nonfinite && bad) atomicAdd(nonfinite, bad);
+  __syncthreads();
+  // pieces: (32-slot block kk, channel c, quarter q) -> 8 values; the
+  // output is 16-byte aligned when the buffer is (a row is 64 bytes)
+  const bool vec = (reinterpret_cast<uintptr_t>(out) & 15u) == 0;
+  const uint32_t npieces = (kGatherSlots / 32) * nc * 4;
+  for (uint32_t p = t; p < npieces; p += 256) {
+    const uint32_t kk = p / (nc * 4), c = (p / 4) % nc, q = p % 4;
+    const uint32_t blk = (i0 >> 5) + kk;
+    if (blk * 32 >= n_pad) continue;
+    const uint16_t* src = lds + c * kGatherRow + kk * 32 + q * 8;
+    T* dst = out + ((size_t)blk * nchan + ch0 + c) * 32 + q * 8;
+    if (vec) {
+      *reinterpret_cast<uint4*>(dst) = *reinterpret_cast<const uint4*>(src);
+    } else {
+#pragma unroll
+      for (uint32_t e = 0; e < 8; ++e) dst[e] = __builtin_bit_cast(T, src[e]);
+    }
+  }
 }
 
 template <typename T>
@@ -755,35 +813,16 @@ __global__ void __launch_bounds__(256) k_gather_channels_cm(uint32_t n, uint32_t
                                                             const uint32_t* __restrict__ perm,
                                                             const T* __restrict__ in, T* __restrict__ out,
                                                             uint32_t* __restrict__ nonfinite) {
-  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= n_pad) return;
-  const uint32_t ch0 = blockIdx.y * kGatherChans;
-  const uint32_t ch1 = min(nchan, ch0 + kGatherChans);
-  uint32_t j = 0;
-  if (i < n) {
-    j = perm[i];
-    if (j >= n) j = i;
-  }
-  uint32_t bad = 0;
-  for (uint32_t ch = ch0; ch < ch1; ++ch) {
-    T v = (T)0.0f;
-    if (i < n) {
-      v = in[(size_t)ch * n + j];
-      bad += isfinite((float)v) ? 0u : 1u;
-    }
-    out[blocked_index(i, ch, nchan)] = v;
-  }
-  if (nonfinite && bad) atomicAdd(nonfinite, bad);
+  __shared__ __attribute__((aligned(16))) uint16_t lds[kGatherChans * kGatherRow];
+  gather_cm_block<T>(blockIdx.x, blockIdx.y, n, n_pad, nchan, perm, in, out, nonfinite, lds);
 }
 
 // Sample-major input ([n][nchan]): 64 x 64 tiles transposed through LDS.
 template <typename T>
-__global__ void __launch_bounds__(256) k_gather_channels_sm(uint32_t n, uint32_t n_pad, uint32_t nchan,
-                                                            const uint32_t* __restrict__ perm,
-                                                            const T* __restrict__ in, T* __restrict__ out,
-                                                            uint32_t* __restrict__ nonfinite) {
-  __shared__ T tile[64][65];
-  const uint32_t i0 = blockIdx.x * 64, ch0 = blockIdx.y * 64;
+__device__ inline void gather_sm_block(uint32_t bx, uint32_t by, uint32_t n, uint32_t n_pad, uint32_t nchan,
+                                       const uint32_t* __restrict__ perm, const T* __restrict__ in,
+                                       T* __restrict__ out, uint32_t* __restrict__ nonfinite, T (*tile)[65]) {
+  const uint32_t i0 = bx * 64, ch0 = by * 64;
   const uint32_t tx = threadIdx.x & 63, ty = threadIdx.x >> 6;  // 64 x 4
   for (uint32_t k = ty; k < 64; k += 4) {
     const uint32_t i = i0 + k, ch = ch0 + tx;
@@ -803,6 +842,102 @@ __global__ void __launch_bounds__(256) k_gather_channels_sm(uint32_t n, uint32_t
   }
 }
 
+template <typename T>
+__global__ void __launch_bounds__(256) k_gather_channels_sm(uint32_t n, uint32_t n_pad, uint32_t nchan,
+                                                            const uint32_t* __restrict__ perm,
+                                                            const T* __restrict__ in, T* __restrict__ out,
+                                                            uint32_t* __restrict__ nonfinite) {
+  __shared__ T tile[64][65];
+  gather_sm_block<T>(blockIdx.x, blockIdx.y, n, n_pad, nchan, perm, in, out, nonfinite, tile);
+}
+
+// Every job's channel gather and tile plan of a batch in one launch
+// (prep_ranges): the plans' dependent key searches are latency-bound and
+// the gathers bandwidth-bound, so sharing the chip lets each hide the
+// other.  A range is a run of workgroups of one kind for one job; each
+// workgroup finds its range by binary search over the range starts.
+__global__ void __launch_bounds__(256) k_batch_prep(const PrepRange* __restrict__ ranges, uint32_t nranges,
+                                                    const PrepJob* __restrict__ pj,
+                                                    const MfmaJob* __restrict__ jobs) {
+  static_assert(kGatherChans * kGatherRow >= 64 * 65, "the sample-major tile fits the channel-major one");
+  __shared__ __attribute__((aligned(16))) uint16_t smem[kGatherChans * kGatherRow];
+  uint16_t(*tile)[65] = reinterpret_cast<uint16_t(*)[65]>(smem);
+  uint32_t lo = 0, hi = nranges;  // last range with first <= blockIdx.x
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (ranges[mid].first <= blockIdx.x)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  const PrepRange R = ranges[lo];
+  const uint32_t lb = blockIdx.x - R.first;
+  const PrepJob& G = pj[R.job];
+  const MfmaJob& J = jobs[R.job];
+  const uint32_t bx = lb % R.gx, by = lb / R.gx;
+  switch (R.kind) {
+    case kPrepPlan: {
+      const uint32_t t = lb * 4 + (threadIdx.x >> 6);  // a wave per tile
+      if (t < J.ntiles) plan_tile(J, t);
+      break;
+    }
+    case kPrepGatherCm:
+      if (J.dtype == HCG_DTYPE_BF16)
+        gather_cm_block<__bf16>(bx, by, G.n, J.n_pad, J.nchan, G.perm, static_cast<const __bf16*>(G.in),
+                                static_cast<__bf16*>(const_cast<void*>(J.vs)), G.nonfinite, smem);
+      else
+        gather_cm_block<_Float16>(bx, by, G.n, J.n_pad, J.nchan, G.perm, static_cast<const _Float16*>(G.in),
+                                  static_cast<_Float16*>(const_cast<void*>(J.vs)), G.nonfinite, smem);
+      break;
+    default:  // kPrepGatherSm
+      if (J.dtype == HCG_DTYPE_BF16)
+        gather_sm_block<__bf16>(bx, by, G.n, J.n_pad, J.nchan, G.perm, static_cast<const __bf16*>(G.in),
+                                static_cast<__bf16*>(const_cast<void*>(J.vs)), G.nonfinite,
+                                reinterpret_cast<__bf16(*)[65]>(tile));
+      else
+        gather_sm_block<_Float16>(bx, by, G.n, J.n_pad, J.nchan, G.perm, static_cast<const _Float16*>(G.in),
+                                  static_cast<_Float16*>(const_cast<void*>(J.vs)), G.nonfinite,
+                                  reinterpret_cast<_Float16(*)[65]>(tile));
+      break;
+  }
+}
+
+int prep_ranges(uint32_t njobs, const MfmaJob* jobs, const uint32_t* layouts, std::vector<PrepRange>& out,
+                uint32_t* blocks) {
+  out.clear();
+  uint64_t first = 0;
+  auto add = [&](uint32_t j, uint32_t kind, uint32_t gx, uint64_t blocks) {
+    if (blocks == 0) return;
+    out.push_back(PrepRange{(uint32_t)first, j, kind, gx});
+    first += blocks;
+  };
+  // plans first: their workgroups run longest
+  for (uint32_t j = 0; j < njobs; ++j)
+    if (jobs[j].plan && jobs[j].nchan) add(j, kPrepPlan, 1, ceil_div_u32(jobs[j].ntiles, 4));
+  for (uint32_t j = 0; j < njobs; ++j) {
+    const MfmaJob& J = jobs[j];
+    if (J.nchan == 0 || J.n_pad == 0) continue;
+    if (layouts[j] == 0) {
+      const uint32_t gx = ceil_div_u32(J.n_pad, kGatherSlots), gy = ceil_div_u32(J.nchan, kGatherChans);
+      add(j, kPrepGatherCm, gx, (uint64_t)gx * gy);
+    } else {
+      const uint32_t gx = ceil_div_u32(J.n_pad, 64), gy = ceil_div_u32(J.nchan, 64);
+      add(j, kPrepGatherSm, gx, (uint64_t)gx * gy);
+    }
+  }
+  if (first >= 0x7FFFFFFFull) return set_error(HCG_ERR_INVALID, "hcg_batch_create: too many prep workgroups");
+  *blocks = (uint32_t)first;
+  return HCG_OK;
+}
+
+int launch_batch_prep(const PrepRange* ranges, uint32_t nranges, uint32_t blocks, const PrepJob* pj,
+                      const MfmaJob* jobs, hipStream_t s) {
+  if (nranges == 0 || blocks == 0) return HCG_OK;
+  k_batch_prep<<<blocks, 256, 0, s>>>(ranges, nranges, pj, jobs);
+  HCG_CHECK_LAUNCH("k_batch_prep");
+  return HCG_OK;
+}
+
 }  // namespace hcg
 
 using namespace hcg;
@@ -818,7 +953,7 @@ extern "C" int hcg_gather_channels(uint32_t n, uint32_t nchan, const uint32_t* p
   const uint32_t n_pad = hcg_channels_padded_len(n);
   hipStream_t s = as_stream(stream);
   if (layout == 0) {
-    const dim3 grid(ceil_div_u32(n_pad, 256), ceil_div_u32(nchan, kGatherChans));
+    const dim3 grid(ceil_div_u32(n_pad, kGatherSlots), ceil_div_u32(nchan, kGatherChans));
     if (dtype == HCG_DTYPE_BF16)
       k_gather_channels_cm<__bf16><<<grid, 256, 0, s>>>(n, n_pad, nchan, perm, static_cast<const __bf16*>(values),
                                                         static_cast<__bf16*>(values_sorted), nonfinite);

@@ -84,7 +84,7 @@ __global__ void __launch_bounds__(64 * S) k_grid_tiles_1ch(GridParams P, GridBuf
 }
 
 // A few-channel job of the multi-channel entry points on the tile path: one
-// wave per tile (few_channel_tile).
+// wave per tile (few_channel_tile_body).
 __global__ void __launch_bounds__(64) k_grid_tiles_few(GridParams P, GridBuffers B, TileOut O, uint32_t tiles_x,
                                                        uint32_t tiles_y) {
   __shared__ __attribute__((aligned(16))) char lds[kFewTileLdsBytes];
@@ -94,7 +94,7 @@ __global__ void __launch_bounds__(64) k_grid_tiles_few(GridParams P, GridBuffers
   const uint32_t ntiles = tiles_x * tiles_y;
   uint32_t tile = blockIdx.x;
   if ((ntiles & 7u) == 0) tile = (blockIdx.x & 7u) * (ntiles >> 3) + (blockIdx.x >> 3);
-  few_channel_tile(P, B, O, tile, tiles_x, lds, s_exp);
+  few_channel_tile_body(P, B, O, tile, tiles_x, lds, s_exp);
 }
 
 int launch_tiles_few_channels(const GridParams& P, const GridBuffers& B, const TileOut& O, hipStream_t s) {

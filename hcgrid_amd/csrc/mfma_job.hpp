@@ -4,6 +4,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "plain_tile.hpp"
 
 namespace hcg {
@@ -49,6 +51,26 @@ struct MfmaItem {
 inline uint32_t tile_shift_of(uint32_t tile_cols) {
   return tile_cols == 8 ? 3u : tile_cols == 16 ? 4u : tile_cols == 32 ? 5u : 6u;
 }
+
+// Batch preparation in one launch: each job's channel gather (input values,
+// permutation and non-finite counter here; output J.vs) and tile plan.
+struct PrepJob {
+  const void* in;          // the caller's channel values (layout per prep_ranges)
+  const uint32_t* perm;    // [n]
+  uint32_t* nonfinite;     // counter, or NULL
+  uint32_t n, pad;
+};
+enum : uint32_t { kPrepPlan = 0, kPrepGatherCm = 1, kPrepGatherSm = 2 };
+struct PrepRange {
+  uint32_t first, job, kind, gx;  // first workgroup; for gathers, workgroups per channel block row
+};
+// The ranges of one prep launch (layouts[j]: job j's value layout, 0
+// channel-major, 1 sample-major; at most 2 per job) and their workgroup
+// total.
+int prep_ranges(uint32_t njobs, const MfmaJob* jobs, const uint32_t* layouts, std::vector<PrepRange>& out,
+                uint32_t* blocks);
+int launch_batch_prep(const PrepRange* ranges, uint32_t nranges, uint32_t blocks, const PrepJob* pj,
+                      const MfmaJob* jobs, hipStream_t s);
 
 // Enqueues the tile-plan kernel of a job into J.plan (non-NULL).
 int launch_mfma_plan(const MfmaJob& J, hipStream_t s);

@@ -450,12 +450,10 @@ __device__ __forceinline__ void one_channel_tile(const GridParams& P, const Grid
   if (wave == 0) tile_write(P, tile, tiles_x, O, d, w);
 }
 
-// One few-channel tile in a kFewTileLdsBytes slice of LDS (one wave).  Out
-// of line: inlined into the persistent batch kernel as a third copy of the
-// tile body, it raised that kernel's register spills and slowed its MFMA
-// and one-channel items by 1-10%; the call costs a few-channel tile ~4%.
-static __device__ __noinline__ void few_channel_tile(const GridParams& P, const GridBuffers& B, const TileOut& O,
-                                                 uint32_t tile, uint32_t tiles_x, char* lds, const double* s_exp) {
+// One few-channel tile in a kFewTileLdsBytes slice of LDS (one wave).
+__device__ __forceinline__ void few_channel_tile_body(const GridParams& P, const GridBuffers& B, const TileOut& O,
+                                                      uint32_t tile, uint32_t tiles_x, char* lds,
+                                                      const double* s_exp) {
   double* s_x = reinterpret_cast<double*>(lds);
   double* s_y = s_x + kFewCap;
   double* s_z = s_y + kFewCap;
@@ -463,6 +461,18 @@ static __device__ __noinline__ void few_channel_tile(const GridParams& P, const 
   uint16_t* s_c = reinterpret_cast<uint16_t*>(s_key + kFewCap);
   grid_tile<HCG_GAUSS1D, false, 2, true, kFewChannels, kFewCap>(P, B, tile, tiles_x, s_x, s_y, s_z, nullptr, s_key,
                                                                 s_exp, O, RingStream{0, 1}, nullptr, nullptr, s_c);
+}
+
+// The same, out of line, for the persistent batch kernel: inlined there as
+// a third copy of the tile body, it raised that kernel's register spills
+// and slowed its MFMA and one-channel items by 1-10%; the call costs a
+// few-channel tile ~4%.  (The one-job kernel inlines the body: a called
+// function takes the register budget of its widest caller.)
+[[maybe_unused]] static __device__ __noinline__ void few_channel_tile(const GridParams& P, const GridBuffers& B,
+                                                                    const TileOut& O, uint32_t tile,
+                                                                    uint32_t tiles_x, char* lds,
+                                                                    const double* s_exp) {
+  few_channel_tile_body(P, B, O, tile, tiles_x, lds, s_exp);
 }
 
 }  // namespace hcg

@@ -394,8 +394,9 @@ size_t hcg_batch_bytes(uint32_t njobs, const hcg_batch_job *jobs, uint32_t waves
 int hcg_batch_create_in(uint32_t njobs, const hcg_batch_job *jobs, uint32_t waves, uint32_t chunk,
                         void *device_mem, size_t bytes, void *stream, hcg_batch **out);
 
-/* Enqueues the batch: counters cleared by a fill kernel, each job's channel
- * gather, then one persistent launch over all work items.  Kernels only
+/* Enqueues the batch: counters cleared by a fill kernel, every job's channel
+ * gather and tile plan in one launch, then one persistent launch over all
+ * work items.  Kernels only
  * (capturable).  Results per job equal hcg_grid_mfma_shaped's with the same
  * tile_cols, bit for bit. */
 int hcg_batch_enqueue(hcg_batch *b, void *stream);

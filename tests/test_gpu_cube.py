@@ -217,3 +217,42 @@ def test_cube_tile_plan_equals_search(gpu, sigma, radius, shape):
     e = map_errors(planned.reshape(-1), o.reshape(-1))
     print(f"  errors {e}", flush=True)
     assert e["nan_match"] and e["max_rel"] <= 1e-4
+
+
+@pytest.mark.parametrize("n,nchan,dtype,layout", [(1000, 5, "bf16", 0), (777, 33, "f16", 0), (1000, 70, "bf16", 1),
+                                                  (64, 1, "f16", 0), (4100, 40, "f16", 1)])
+def test_gather_channels_exact(gpu, n, nchan, dtype, layout):
+    # the blocked layout, element for element: slot i of channel ch at
+    # ((i // 32) * nchan + ch) * 32 + i % 32, zeros past n, and a count of the
+    # non-finite values among the n samples (planted NaNs and infinities)
+    import ctypes as C
+
+    import torch
+
+    from hcgrid_amd import capi
+    rng = np.random.default_rng(n + nchan)
+    perm = rng.permutation(n).astype(np.uint32)
+    v = rng.standard_normal((nchan, n)).astype(np.float32)
+    bad = [(0, 3), (nchan - 1, n - 1), (nchan // 2, n // 2)]
+    for k, (ch, i) in enumerate(bad):
+        v[ch, i] = np.nan if k % 2 == 0 else np.inf
+    tdt = torch.bfloat16 if dtype == "bf16" else torch.float16
+    vt = torch.tensor(v).to(tdt)
+    src = (vt if layout == 0 else vt.t()).contiguous().cuda()
+    n_pad = capi.load().hcg_channels_padded_len(n)
+    out = torch.full((n_pad * nchan,), 7.0, dtype=tdt, device="cuda")  # every element must be written
+    cnt = torch.zeros(1, dtype=torch.int32, device="cuda")
+    pd = torch.tensor(perm.astype(np.int32), device="cuda")
+    print(f"case: gather n={n} nchan={nchan} {dtype} layout={layout}", flush=True)
+    capi.call("hcg_gather_channels", n, nchan, C.c_void_p(pd.data_ptr()), C.c_void_p(src.data_ptr()),
+              capi.DTYPE_BF16 if dtype == "bf16" else capi.DTYPE_F16, layout, C.c_void_p(out.data_ptr()),
+              C.c_void_p(cnt.data_ptr()), None)
+    torch.cuda.synchronize()
+    got = out.view(torch.int16).cpu().numpy()
+    bits = vt.view(torch.int16).numpy()  # [nchan][n]
+    want = np.zeros(n_pad * nchan, dtype=np.int16)
+    i = np.arange(n)
+    for ch in range(nchan):
+        want[((i // 32) * nchan + ch) * 32 + i % 32] = bits[ch, perm]
+    assert np.array_equal(got, want)
+    assert int(cnt.item()) == len(bad)
