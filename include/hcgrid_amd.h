@@ -395,8 +395,10 @@ int hcg_batch_create_in(uint32_t njobs, const hcg_batch_job *jobs, uint32_t wave
                         void *device_mem, size_t bytes, void *stream, hcg_batch **out);
 
 /* Enqueues the batch: counters cleared by a fill kernel, every job's channel
- * gather and tile plan in one launch, then one persistent launch over all
- * work items.  Kernels only
+ * gather and tile plan in one launch, then the persistent launch over the
+ * work items (dense jobs, 32+ samples per cell, in a persistent launch of
+ * their own at 8 waves / chunk 512 first, when the batch's shape is not
+ * that one).  Kernels only
  * (capturable).  Results per job equal hcg_grid_mfma_shaped's with the same
  * tile_cols, bit for bit. */
 int hcg_batch_enqueue(hcg_batch *b, void *stream);
