@@ -237,12 +237,15 @@ extern "C" size_t hcg_batch_job_bytes(const hcg_geometry* g, uint32_t nchan) {
 namespace {
 
 // A batch's plan: work items and the layout of its one device allocation.
-// Dense MFMA jobs (at least kDenseSamplesPerCell samples per map cell) run
-// in a persistent launch of their own at 8 waves and 512-sample chunks, the
-// shape that suits them (one workgroup per CU sharing each block's weights
-// over 512 channels): a 3.9M-sample 128^2 x 512-channel job takes 18.2 ms
-// there against 26.7 ms at 4 waves / 256, while sparse jobs run up to 1.8x
-// slower at 8 waves.  Results do not depend on the launch shape.
+// Dense MFMA jobs (at least kDenseSamplesPerCell samples per map cell), and
+// jobs with more channels than one channel group of the batch's shape
+// holds, run in a persistent launch of their own at 8 waves and 512-sample
+// chunks, the shape that suits them (one workgroup per CU sharing each
+// block's weights over 512 channels): a 3.9M-sample 128^2 x 512-channel job
+// takes 18.2 ms there against 26.7 ms at 4 waves / 256, a 3.2M-sample
+// 2048^2 x 512-channel one 21.6 against 24.5, while sparse jobs with one
+// channel group run up to 1.8x slower at 8 waves.  Results do not depend on
+// the launch shape.
 constexpr double kDenseSamplesPerCell = 32.0;
 constexpr uint32_t kDenseWaves = 8, kDenseChunk = 512;
 
@@ -315,7 +318,10 @@ int plan_batch(uint32_t njobs, const hcg_batch_job* jobs, uint32_t waves, uint32
       const uint32_t tc = b.tile_cols ? b.tile_cols : kMcDefaultTileCols;
       p.tcols[j] = tc;
       p.tiles[j] = (b.nchan && side) ? ceil_div_u32(side, tc) * ceil_div_u32(side, 64 / tc) : 0;
-      dense[j] = waves != kDenseWaves && side && (double)g->n >= kDenseSamplesPerCell * side * side;
+      // dense, or more channels than one channel group at the batch's shape
+      // covers (each group recomputes the block's weights)
+      dense[j] = waves != kDenseWaves && side &&
+                 ((double)g->n >= kDenseSamplesPerCell * side * side || b.nchan > waves * kMcChanPerWave);
       const uint32_t w = dense[j] ? kDenseWaves : waves;
       groups[j] = ceil_div_u32(ceil_div_u32(b.nchan, kMcChanPerWave), w);
       units[j] = p.tiles[j];
