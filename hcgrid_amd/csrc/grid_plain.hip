@@ -17,9 +17,13 @@
 //    ang2pix for every candidate (gridding.cu:373; equal by construction);
 //  * distance: h = |p - q|^2 / 4 = sin^2(d/2) from precomputed unit vectors
 //    (the haversine quantity of helpers.cu:128-130, exact in real
-//    arithmetic, ~1e-12 relative in FP64).  Pairs with h within 1e-9
-//    (relative) of the radius threshold are decided with the reference's
-//    haversine formula itself; d^2 for the Gaussian comes from the series
+//    arithmetic, ~1e-12 relative in FP64); for GAUSS1D at radii with
+//    c2 threshold >= 1e-7 (dot_distance_ok) c2 = 2 - 2 p.q instead, which
+//    tests a candidate in four FP64 operations rather than seven and is
+//    good to ~1.2e-15 absolute (weights to ~5e-8 relative at worst).  Pairs
+//    within the guard band of the radius threshold (1e-9 relative + 4e-15
+//    absolute) are decided with the reference's haversine formula itself,
+//    so selection is exact in both forms; d^2 for the Gaussian comes from the series
 //    4 asin^2(sqrt h) = 4 (h + h^2/3 + 8h^3/45 + 4h^4/35 + ...), exact to
 //    <1e-17 relative for h < 1e-4 (radius < 1.1 deg), asin otherwise.
 //
@@ -275,7 +279,11 @@ int launch_grid(const GridParams& P, const GridBuffers& B, hipStream_t s) {
     if (blocks == 0) return HCG_OK;
     // GAUSS1D with a radius whose pairs all take the exponent's series branch
     // (and stay clear of the exp clamp): the forms without either check
-    if (KT == HCG_GAUSS1D && HCG_GRID_SMALLR && gauss_tiny_radius(P))
+    if (KT == HCG_GAUSS1D && HCG_GRID_SMALLR && gauss_tiny_radius(P) && dot_distance_ok(P) &&
+        kCellsPerLane == 1 && !HCG_GRID_LANE_LOOP)
+      k_grid_tiles<KT, HAS_W, (KT == HCG_GAUSS1D && kCellsPerLane == 1 && !HCG_GRID_LANE_LOOP) ? 3 : 2>
+          <<<(uint32_t)blocks, 64, 0, s>>>(P, B, tiles_x, tiles_y);
+    else if (KT == HCG_GAUSS1D && HCG_GRID_SMALLR && gauss_tiny_radius(P))
       k_grid_tiles<KT, HAS_W, 2><<<(uint32_t)blocks, 64, 0, s>>>(P, B, tiles_x, tiles_y);
     else if (KT == HCG_GAUSS1D && HCG_GRID_SMALLR && gauss_small_radius(P))
       k_grid_tiles<KT, HAS_W, 1><<<(uint32_t)blocks, 64, 0, s>>>(P, B, tiles_x, tiles_y);

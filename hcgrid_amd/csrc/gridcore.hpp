@@ -227,6 +227,14 @@ inline bool gauss_small_radius(const GridParams& P) {
   return y > -90000.0;
 }
 
+// The tile loop's candidate test on p.q (c2 = 2 - 2 p.q, four FP64 operations
+// instead of seven for |p - q|^2): its c2 carries ~1.2e-15 absolute error,
+// which at c_hi >= 1e-7 (R >= ~0.018 deg) is <= 1.2e-8 of the threshold, so
+// a weight exp(-p0 d^2) (p0 d^2 <= 4.5 at R = 3 sigma) moves by <= ~5e-8
+// relative; smaller radii keep |p - q|^2.  Selection is exact either way
+// (the guard band covers the error and the reference's formula decides).
+inline bool dot_distance_ok(const GridParams& P) { return P.c_hi >= 1e-7; }
+
 // gauss_small_radius and c_hi < 2e-5 (R < ~0.26 deg), where the series' c2^3
 // term, (4/560) c2^3 relative to the leading one, is below 6e-17 (host side).
 inline bool gauss_tiny_radius(const GridParams& P) { return gauss_small_radius(P) && P.c_hi < 2e-5; }
@@ -330,12 +338,14 @@ inline GridParams make_grid_params(const hcg_healpix* hp, const hcg_kernel* kern
   P.side = side;
   P.selection = kern->selection;
   P.factor = kern->selection == HCG_SELECT_DISC ? 1u : kern->factor;
-  // radius threshold on c2 = 4 sin^2(R/2), with a 1e-9 relative guard band
-  // where the reference's haversine decides (the unit-vector c2 is good to ~1e-12)
+  // radius threshold on c2 = 4 sin^2(R/2), with a guard band where the
+  // reference's haversine decides: 1e-9 relative (c2 = |p - q|^2 is good to
+  // ~1e-12 relative) plus 4e-15 absolute (c2 = 2 - 2 p.q, the form the tile
+  // loop uses at dot_distance_ok radii, is good to ~1.2e-15 absolute)
   const double half = kern->sphere_radius * kDeg2Rad / 2.0;
   const double cR = 4.0 * sin(half) * sin(half);
-  P.c_lo = cR * (1.0 - 1e-9);
-  P.c_hi = cR * (1.0 + 1e-9);
+  P.c_lo = cR * (1.0 - 1e-9) - 4e-15;
+  P.c_hi = cR * (1.0 + 1e-9) + 4e-15;
   // -p0 d^2 = -p0 (2 RAD2DEG)^2 asin^2(sqrt(c2)/2); in units of ln2/64
   const double inv_ln2_64 = 0x1.71547652b82fep+6;
   P.k_asin = -4.0 * kRad2Deg * kRad2Deg * kern->params[0] * inv_ln2_64;

@@ -55,6 +55,7 @@
 #define HCG_GRID_TABLE_EXP 1
 #endif
 
+
 namespace hcg {
 
 constexpr int kTileCols = HCG_GRID_TILE_COLS;
@@ -153,7 +154,8 @@ int launch_tiles_one_channel(const GridParams& P, const GridBuffers& B, const Ti
 // Launches the tile path for one few-channel job: one wave per tile.
 int launch_tiles_few_channels(const GridParams& P, const GridBuffers& B, const TileOut& O, hipStream_t s);
 
-// WMODE (GAUSS1D): 0 general exponent and exp; 1 gauss_small_radius; 2 gauss_tiny_radius.
+// WMODE (GAUSS1D): 0 general exponent and exp; 1 gauss_small_radius; 2 gauss_tiny_radius; 3 as 2,
+// with the candidate test on p.q (dot_distance_ok).
 // s_exp: the 64-entry exponential table (exp_table_load), visible to the wave.
 //
 // NC > 0 (few-channel tiles, BATCH true): up to NC channels of 16-bit values
@@ -168,6 +170,8 @@ __device__ __forceinline__ void grid_tile(const GridParams P, const GridBuffers 
                                           const TileOut& O, RingStream S, double* sum_d, double* sum_w,
                                           uint16_t* __restrict__ s_c = nullptr) {
   static_assert(!BATCH || kCellsPerLane == 1, "the one-channel path keeps one cell per lane");
+  static_assert(WMODE != 3 || (kCellsPerLane == 1 && !HCG_GRID_LANE_LOOP && KT == HCG_GAUSS1D),
+                "the p.q candidate test is written for the one-cell-per-lane GAUSS1D loop");
   static_assert(NC == 0 || (BATCH && NC % 2 == 0), "few-channel tiles: an even channel count, batch outputs");
   float acc_c[NC > 0 ? NC : 1];
 #pragma unroll
@@ -305,14 +309,24 @@ __device__ __forceinline__ void grid_tile(const GridParams P, const GridBuffers 
       // one cell per lane: its band is exactly [lo, lo + len)
       (void)jb;
       (void)je;
+      const double dot_lo = fma(-0.5, P.c_hi, 1.0);  // c2 < c_hi  <=>  p.q > 1 - c_hi / 2
+      (void)dot_lo;
       for (uint32_t t = 0; t < len[0]; ++t) {
         const uint32_t j = lo[0] + t;
         const double qx = s_x[j], qy = s_y[j], qz = s_z[j], qv = NC > 0 ? 0.0 : s_v[j];
         {
           constexpr int k = 0;
-          const double dx = px[k] - qx, dy = py[k] - qy, dz = pz[k] - qz;
-          const double c2 = fma(dx, dx, fma(dy, dy, dz * dz));
-          if (c2 < P.c_hi) {
+          // WMODE 3: the candidate test on p.q and c2 = 2 - 2 p.q (four FP64
+          // operations instead of seven); otherwise c2 = |p - q|^2
+          double c2, dot = 0.0;
+          if constexpr (WMODE == 3) {
+            dot = fma(px[k], qx, fma(py[k], qy, pz[k] * qz));
+          } else {
+            const double dx = px[k] - qx, dy = py[k] - qy, dz = pz[k] - qz;
+            c2 = fma(dx, dx, fma(dy, dy, dz * dz));
+          }
+          if (WMODE == 3 ? dot > dot_lo : c2 < P.c_hi) {
+            if constexpr (WMODE == 3) c2 = fma(-2.0, dot, 2.0);
 #else
       for (uint32_t j = jb; j < je; ++j) {
         const double qx = s_x[j], qy = s_y[j], qz = s_z[j], qv = s_v[j];
@@ -325,7 +339,7 @@ __device__ __forceinline__ void grid_tile(const GridParams P, const GridBuffers 
             if (c2 < P.c_lo || radius_test_ref(B.xwcs, B.ywcs, B.lon_s, B.lat_s, cell[k], cb + j, P.sphere_radius)) {
               double wt;
               if constexpr (KT == HCG_GAUSS1D && WMODE > 0)
-                wt = gauss_weight_series<WMODE>(P, c2, s_exp);
+                wt = gauss_weight_series<WMODE == 3 ? 2 : WMODE>(P, c2, s_exp);
               else if constexpr (KT == HCG_GAUSS1D)
                 wt = HCG_GRID_TABLE_EXP ? exp_scaled_neg(gauss_exponent_scaled(P, c2), s_exp)
                                         : exp(gauss_exponent_scaled(P, c2) * 0x1.62e42fefa39efp-7);

@@ -133,3 +133,12 @@ def test_weight_paths_by_radius(gpu, sigma, radius):
     wl = synth.raster_small(40, 120, 24, spacing_dec=0.05, spacing_ra=0.01, cdelt=0.05, seed=31)
     xw, yw = _target_coords(wl.target)
     _compare(wl.lon, wl.lat, wl.values, xw, yw, wl.target.side, sigma=sigma, radius=radius, res=sigma / 2)
+
+
+def test_tiny_radius_difference_form(gpu):
+    # R = 0.015 deg puts the c2 threshold (6.9e-8) under the 1e-7 floor of the
+    # p.q candidate test, so the tile loop keeps c2 = |p - q|^2 here; the
+    # other tests (R >= 0.045 deg) run the p.q form
+    wl = synth.raster_small(40, 200, 24, spacing_dec=0.004, spacing_ra=0.002, seed=37)
+    xw, yw = _target_coords(wl.target)
+    _compare(wl.lon, wl.lat, wl.values, xw, yw, wl.target.side, sigma=0.005, radius=0.015, res=0.01)
