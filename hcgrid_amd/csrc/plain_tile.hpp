@@ -132,7 +132,8 @@ inline uint32_t one_channel_streams(uint32_t side) {
 }
 
 // Few-channel jobs (2..kFewChannels channels, otherwise as one_channel_tiles,
-// on maps of at least kOneChannelMinTiles tiles) also take the tile loop:
+// at dot_distance_ok radii (their tile loop tests candidates on p.q), on
+// maps of at least kOneChannelMinTiles tiles) also take the tile loop:
 // FP64 weights, FP32 accumulation per channel.  The MFMA kernel pays a
 // fixed chain of barriers and global round trips per 64-cell block, which
 // on sparse large maps outweighs the per-pair work (300k samples x 16
@@ -144,7 +145,7 @@ constexpr size_t kFewTileLdsBytes = (size_t)kFewCap * (3 * sizeof(double) + size
 
 inline bool few_channel_tiles(const GridParams& P, uint32_t kernel_type, bool weighted, uint32_t nchan) {
   return nchan >= 2 && nchan <= (uint32_t)kFewChannels && kernel_type == HCG_GAUSS1D && !weighted &&
-         gauss_tiny_radius(P) && tile_count_of(P.side) >= kOneChannelMinTiles;
+         gauss_tiny_radius(P) && dot_distance_ok(P) && tile_count_of(P.side) >= kOneChannelMinTiles;
 }
 
 // Launches the tile path for one one-channel job: one workgroup per tile,
@@ -473,7 +474,7 @@ __device__ __forceinline__ void few_channel_tile_body(const GridParams& P, const
   double* s_z = s_y + kFewCap;
   uint32_t* s_key = reinterpret_cast<uint32_t*>(s_z + kFewCap);
   uint16_t* s_c = reinterpret_cast<uint16_t*>(s_key + kFewCap);
-  grid_tile<HCG_GAUSS1D, false, 2, true, kFewChannels, kFewCap>(P, B, tile, tiles_x, s_x, s_y, s_z, nullptr, s_key,
+  grid_tile<HCG_GAUSS1D, false, 3, true, kFewChannels, kFewCap>(P, B, tile, tiles_x, s_x, s_y, s_z, nullptr, s_key,
                                                                 s_exp, O, RingStream{0, 1}, nullptr, nullptr, s_c);
 }
 
