@@ -256,3 +256,36 @@ def test_gather_channels_exact(gpu, n, nchan, dtype, layout):
         want[((i // 32) * nchan + ch) * 32 + i % 32] = bits[ch, perm]
     assert np.array_equal(got, want)
     assert int(cnt.item()) == len(bad)
+
+
+@pytest.mark.parametrize("side,nchan", [(24, 1), (512, 1), (512, 16), (24, 48)])
+def test_raw_sums_match_normalised(gpu, side, nchan):
+    # normalize = 0 on every route (one channel on 4 ring streams and on one
+    # wave per tile, few-channel tiles, the MFMA contraction): the raw cube
+    # is the normalised cube times the weight sum, and 0 where no sample
+    # contributes (NaN once normalised); the weight sum is the same either way
+    import torch
+
+    from hcgrid_amd import GAUSS1D
+    wl = synth.raster_small(40, 300 if side >= 512 else 200, side, seed=90 + nchan)
+    t = wl.target
+    xw, yw = gridder.wcs_sin(side, side, t.crpix, t.crpix, -t.cdelt, t.cdelt, t.crval1, t.crval2)
+    V = torch.tensor(_values(wl, nchan, 5)).to(torch.bfloat16).cuda()
+    kern, hp = gridder.prepare_grid_kernel(GAUSS1D, [wl.sigma], wl.radius, wl.hpx_res)
+    g = gridder.CubeGridder(wl.n, side, nchan, kern, hp, dtype="bf16")
+    d = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device="cuda")  # noqa: E731
+    print(f"case: raw sums side={side} nchan={nchan}", flush=True)
+    out = []
+    for norm in (True, False):
+        g.run(d(wl.lon), d(wl.lat), V, d(xw), d(yw), normalize=norm)
+        torch.cuda.synchronize()
+        out.append((g.cube.float().cpu().numpy().reshape(nchan, -1).copy(), g.wsum.cpu().numpy().copy()))
+    (cn, wn), (cr, wr) = out
+    assert np.array_equal(wn, wr)
+    has = wn > 0
+    assert has.any()
+    assert np.all(np.isnan(cn[:, ~has])) and np.all(cr[:, ~has] == 0)  # empty cells (the 512^2 maps)
+    if side >= 512:
+        assert (~has).any()
+    scale = np.abs(cr[:, has]).max()
+    np.testing.assert_allclose(cr[:, has], cn[:, has] * wn[has], rtol=1e-5, atol=1e-6 * scale)
