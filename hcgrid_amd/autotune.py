@@ -1,12 +1,20 @@
 """Launch-shape autotuner for the multi-channel gridder.
 
-Two knobs, both without effect on results (the tests check bit equality):
+Two knobs:
   * per job class, the block shape `tile_cols` (cells per workgroup form a
     (64 / tile_cols) x tile_cols block of the map; squarer blocks re-read
-    fewer channel values, wider ones cross fewer rings);
+    fewer channel values, wider ones cross fewer rings).  The FP32
+    accumulation order of the contraction follows the block shape, so two
+    tile_cols give cubes equal only up to FP32 summation order (both within
+    the stated bound of the FP64 oracle), not bit for bit;
   * per batch, the persistent launch's `waves` per workgroup and `chunk`
     (samples staged per step), which fix how many channels one work item
-    carries (waves x 64) and how many workgroups share a CU.
+    carries (waves x 64) and how many workgroups share a CU.  These leave
+    every result bit-identical (the tests check it).
+
+Because the tuner picks tile_cols from timings, two fresh tuning runs can
+pick different shapes and so differ in the last FP32 bits.  Persist the
+choices (`path`) and reuse them to keep results reproducible between runs.
 
 A job class is (channel bucket, sigma, value type, log2 samples-per-cell
 bucket): jobs in one class have the same support in cells and the same

@@ -16,9 +16,46 @@ import numpy as np
 
 from . import capi
 from .capi import BatchJob, BatchStats, GeometryView, call
-from .gridder import _ptr, _stream_handle, _torch
+from .gridder import _ptr, _stream_handle, _torch, check_device_of, check_tensor
 
 DTYPES = {"bf16": capi.DTYPE_BF16, "f16": capi.DTYPE_F16}
+
+
+def validate_job(i: int, j: dict) -> int:
+    """Checks one Batch job dict before any pointer of it reaches the C ABI;
+    returns its channel count.  values: contiguous [nchan][n] tensor of the
+    declared dtype; cube: contiguous float32 [nchan][side*side]; wsum: None
+    or contiguous float32 [side*side]; all on the geometry's device (any GPU
+    when the geometry does not say)."""
+    where = f"Batch job {i}"
+    g = j["geometry"]
+    dtype = j.get("dtype", "bf16")
+    if dtype not in DTYPES:
+        raise capi.HcgError(capi.HCG_ERR_INVALID, where, f"dtype {dtype!r} (expected 'bf16' or 'f16')")
+    vals = j.get("values")
+    device = getattr(g, "device", None)
+    m = int(g.side) * int(g.side)
+    if vals is None:
+        return 0
+    if vals.dim() != 2 or int(vals.shape[1]) != int(g.n):
+        raise capi.HcgError(capi.HCG_ERR_INVALID, where,
+                            f"values have shape {tuple(vals.shape)}, expected (nchan, {g.n})")
+    nchan = int(vals.shape[0])
+    check_tensor(vals, where, "values", dtype, nchan * int(g.n), check_device=False)
+    if nchan == 0:
+        return 0
+    cube = j.get("cube")
+    if cube is None or tuple(cube.shape) not in ((nchan, m), (nchan * m,)):
+        raise capi.HcgError(capi.HCG_ERR_INVALID, where,
+                            f"cube shape {None if cube is None else tuple(cube.shape)}, expected ({nchan}, {m})")
+    check_tensor(cube, where, "cube", "f32", nchan * m, check_device=False)
+    w = j.get("wsum")
+    if w is not None:
+        check_tensor(w, where, "wsum", "f32", m, check_device=False)
+    for name, t in (("values", vals), ("cube", cube), ("wsum", w)):
+        if t is not None:
+            check_device_of(t, where, name, device)
+    return nchan
 
 
 class Geometry:
@@ -28,12 +65,13 @@ class Geometry:
         """in_torch: place the geometry in a torch allocation (the caching
         allocator), so creating and dropping it makes no hipMalloc/hipFree."""
         capi.load()
+        torch = _torch()
         self.kern, self.hp = kern, hp
         self.n, self.side = int(n), int(side)
+        self.device = torch.device("cuda", torch.cuda.current_device())  # where hcg_geometry_* allocates
         self._h = C.c_void_p()
         self._mem = None
         if in_torch:
-            torch = _torch()
             nb = Geometry.bytes_for(self.n, self.side, hp)
             self._mem = torch.empty(nb + 256, dtype=torch.uint8, device="cuda")
             base = (self._mem.data_ptr() + 255) // 256 * 256
@@ -67,12 +105,23 @@ class Geometry:
         yp = C.c_void_p(self.view.ywcs) if ywcs is None else _ptr(ywcs)
         call("hcg_geometry_build", self.handle, _ptr(lon), _ptr(lat), xp, yp, _stream_handle(stream))
 
-    def set_target_sin(self, target, stream=None):
-        """Fills the geometry's target coordinates on the device: a SIN map
-        of target.side^2 cells (hcg_wcs_sin_p2s_device)."""
+    def set_target(self, target, stream=None):
+        """Fills the geometry's target coordinates on the device: a zenithal
+        map (target.proj, default SIN) of target.side^2 cells, CDELT1 =
+        -cdelt, CDELT2 = cdelt, LONPOLE 0 at CRVAL2 >= 90 and 180 otherwise
+        (hcg_wcs_p2s_device)."""
         t = target
-        call("hcg_wcs_sin_p2s_device", self.side, self.side, t.crpix, t.crpix, -t.cdelt, t.cdelt, t.crval1,
-             t.crval2, 180.0, C.c_void_p(self.view.xwcs), C.c_void_p(self.view.ywcs), _stream_handle(stream))
+        if t.side != self.side:
+            raise capi.HcgError(capi.HCG_ERR_INVALID, "Geometry.set_target", f"side {t.side} != {self.side}")
+        proj = getattr(t, "proj", "SIN")
+        lonpole = 0.0 if t.crval2 >= 90.0 else 180.0
+        call("hcg_wcs_p2s_device", capi.PROJECTIONS[proj], self.side, self.side, t.crpix, t.crpix, -t.cdelt,
+             t.cdelt, t.crval1, t.crval2, lonpole, C.c_void_p(self.view.xwcs), C.c_void_p(self.view.ywcs),
+             _stream_handle(stream))
+
+    def set_target_sin(self, target, stream=None):
+        """set_target for a SIN map (the configured workloads' projection)."""
+        self.set_target(target, stream)
 
     def _read(self, ptr, count, dtype):
         torch = _torch()
@@ -118,17 +167,13 @@ class Batch:
         synchronisation); the batch must then be enqueued on that stream."""
         capi.load()
         self.jobs = list(jobs)
+        nchans = [validate_job(i, j) for i, j in enumerate(self.jobs)]
         arr = (BatchJob * max(1, len(self.jobs)))()
         for i, j in enumerate(self.jobs):
             g = j["geometry"]
             vals = j["values"]
-            nchan = int(vals.shape[0]) if vals is not None else 0
-            if vals is not None and int(vals.shape[1]) != g.n:
-                raise capi.HcgError(capi.HCG_ERR_INVALID, "Batch", f"job {i}: values have {vals.shape[1]} samples, "
-                                    f"geometry {g.n}")
+            nchan = nchans[i]
             cube = j["cube"]
-            if nchan and tuple(cube.shape) != (nchan, g.side * g.side):
-                raise capi.HcgError(capi.HCG_ERR_INVALID, "Batch", f"job {i}: cube shape {tuple(cube.shape)}")
             arr[i].geometry = g.handle.value
             arr[i].nchan = nchan
             arr[i].dtype = DTYPES[j.get("dtype", "bf16")]
@@ -161,6 +206,17 @@ class Batch:
 
     def enqueue(self, stream=None):
         call("hcg_batch_enqueue", self.handle, _stream_handle(stream))
+
+    def timing(self, enable: bool = True):
+        """Records HIP events around the prep and gridding launches of every
+        later eager enqueue (hcg_batch_timing)."""
+        call("hcg_batch_timing", self.handle, 1 if enable else 0)
+
+    def last_times(self) -> tuple[float, float]:
+        """(prep_ms, grid_ms) of the last timed enqueue; waits for it."""
+        p, g = C.c_float(), C.c_float()
+        call("hcg_batch_last_times", self.handle, C.byref(p), C.byref(g))
+        return float(p.value), float(g.value)
 
     def stats(self) -> dict:
         st = BatchStats()

@@ -32,13 +32,14 @@ constexpr int kSortWaves = kSortThreads / 64;
 __global__ void __launch_bounds__(256) k_keys(hcg_healpix hp, uint32_t n, const double* __restrict__ lon,
                                               const double* __restrict__ lat, uint32_t* __restrict__ keys,
                                               uint32_t* __restrict__ idx) {
-  const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+  // 64-bit index: with a 32-bit one, i + stride wraps below n for n close to 2^32
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
     // HCGrid/gridding.cu:78-80
     const double theta = kHalfPi - kDeg2Rad * lat[i];
     const double phi = kDeg2Rad * lon[i];
     keys[i] = (uint32_t)ang2pix(hp, theta, phi);
-    idx[i] = i;
+    idx[i] = (uint32_t)i;
   }
 }
 
@@ -180,10 +181,10 @@ __global__ void __launch_bounds__(256) k_gather(uint32_t n, const uint32_t* __re
                                                 const double* __restrict__ w, hcg_sample* __restrict__ rec,
                                                 double* __restrict__ lon_s, double* __restrict__ lat_s,
                                                 double* __restrict__ data_s, double* __restrict__ w_s) {
-  const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
     uint32_t j = perm[i];
-    if (j >= n) j = i;  // a permutation never holds this; keeps every read in bounds regardless
+    if (j >= n) j = (uint32_t)i;  // a permutation never holds this; keeps every read in bounds regardless
     const double lo = lon[j], la = lat[j];
     const double v = !data                      ? 0.0  // multi-channel jobs carry values separately
                      : (dtype == HCG_DTYPE_F64) ? static_cast<const double*>(data)[j]

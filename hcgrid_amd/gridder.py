@@ -69,6 +69,45 @@ def _ptr(t) -> C.c_void_p | None:
     return None if t is None else C.c_void_p(t.data_ptr())
 
 
+_DTYPE_NAMES = {"f64": "float64", "f32": "float32", "bf16": "bfloat16", "f16": "float16", "i32": "int32"}
+
+
+def check_tensor(t, where: str, name: str, dtypes, numel: int | None = None, device=None, min_numel=None,
+                 check_device: bool = True):
+    """Raises HcgError unless `t` is a contiguous torch tensor of one of
+    `dtypes` (names: "f64", "f32", "bf16", "f16", "i32") with `numel`
+    elements (or at least `min_numel`) on `device` (a torch.device; None =
+    any CUDA device).  Called before any pointer goes to the C ABI, which
+    sees only addresses and would read or write out of bounds otherwise.
+    Dtype, size and layout are checked before the device, so the checks
+    can be exercised with CPU tensors."""
+    torch = _torch()
+    if not isinstance(t, torch.Tensor):
+        raise capi.HcgError(capi.HCG_ERR_INVALID, where, f"{name} must be a torch tensor, got {type(t).__name__}")
+    if isinstance(dtypes, str):
+        dtypes = (dtypes,)
+    allowed = [getattr(torch, _DTYPE_NAMES[d]) for d in dtypes]
+    if t.dtype not in allowed:
+        raise capi.HcgError(capi.HCG_ERR_INVALID, where,
+                            f"{name} has dtype {t.dtype}, expected {' or '.join(str(a) for a in allowed)}")
+    if numel is not None and t.numel() != numel:
+        raise capi.HcgError(capi.HCG_ERR_INVALID, where, f"{name} has {t.numel()} elements, expected {numel}")
+    if min_numel is not None and t.numel() < min_numel:
+        raise capi.HcgError(capi.HCG_ERR_INVALID, where,
+                            f"{name} has {t.numel()} elements, expected at least {min_numel}")
+    if not t.is_contiguous():
+        raise capi.HcgError(capi.HCG_ERR_INVALID, where, f"{name} is not contiguous")
+    if check_device:
+        check_device_of(t, where, name, device)
+
+
+def check_device_of(t, where: str, name: str, device=None):
+    """Raises HcgError unless tensor `t` is on `device` (None = any GPU)."""
+    if t.device.type != "cuda" or (device is not None and t.device != device):
+        want = device if device is not None else "a GPU"
+        raise capi.HcgError(capi.HCG_ERR_INVALID, where, f"{name} is on {t.device}, expected {want}")
+
+
 def _stream_handle(stream) -> C.c_void_p:
     torch = _torch()
     s = stream if stream is not None else torch.cuda.current_stream()
@@ -128,8 +167,13 @@ class PlainGridder:
         self.ws_bytes = ws_bytes
 
     # -- stages ------------------------------------------------------------
+    def _check_coords(self, where, lon, lat):
+        check_tensor(lon, where, "lon", "f64", self.n, self.device)
+        check_tensor(lat, where, "lat", "f64", self.n, self.device)
+
     def presort(self, lon, lat, stream=None):
         b = self.buf
+        self._check_coords("presort", lon, lat)
         call("hcg_presort", C.byref(self.hp), self.n, _ptr(lon), _ptr(lat), _ptr(b.keys), _ptr(b.perm),
              _ptr(b.workspace), self.ws_bytes, _stream_handle(stream))
 
@@ -137,10 +181,12 @@ class PlainGridder:
         torch = _torch()
         b = self.buf
         dtype = capi.DTYPE_F64 if data.dtype == torch.float64 else capi.DTYPE_F32
-        if data.dtype not in (torch.float64, torch.float32):
-            raise capi.HcgError(capi.HCG_ERR_INVALID, "gather", "data must be float32 or float64")
+        self._check_coords("gather", lon, lat)
+        check_tensor(data, "gather", "data", ("f32", "f64"), self.n, self.device)
         if weights is not None and b.w_s is None:
             raise capi.HcgError(capi.HCG_ERR_INVALID, "gather", "gridder built without weights")
+        if weights is not None:
+            check_tensor(weights, "gather", "weights", "f64", self.n, self.device)
         call("hcg_gather_samples", self.n, _ptr(b.perm), _ptr(lon), _ptr(lat), _ptr(data), dtype,
              _ptr(weights), _ptr(b.records), _ptr(b.lon_s), _ptr(b.lat_s), None,
              _ptr(b.w_s) if weights is not None else None, _stream_handle(stream))
@@ -159,6 +205,9 @@ class PlainGridder:
 
     def grid(self, xwcs, ywcs, weighted=False, stream=None):
         b = self.buf
+        m = self.side * self.side
+        check_tensor(xwcs, "grid", "xwcs", "f64", m, self.device)
+        check_tensor(ywcs, "grid", "ywcs", "f64", m, self.device)
         call("hcg_grid_plain", C.byref(self.hp), C.byref(self.kern), self.side, self.n, _ptr(b.records),
              _ptr(b.keys), _ptr(b.lon_s), _ptr(b.lat_s), _ptr(b.w_s) if weighted else None, _ptr(b.lut),
              _ptr(b.ring_z), _ptr(b.ring_s), _ptr(b.meta), _ptr(xwcs), _ptr(ywcs), _ptr(b.datacube),
@@ -249,6 +298,13 @@ class CubeGridder:
         self._ws = None
 
     def gather_channels(self, values, layout: int = 0, stream=None):
+        if layout not in (0, 1):
+            raise capi.HcgError(capi.HCG_ERR_INVALID, "gather_channels", "layout must be 0 or 1")
+        check_tensor(values, "gather_channels", "values", self.dtype, self.nchan * self.n, self.plain.device)
+        shape = (self.nchan, self.n) if layout == 0 else (self.n, self.nchan)
+        if values.dim() == 2 and tuple(values.shape) != shape:
+            raise capi.HcgError(capi.HCG_ERR_INVALID, "gather_channels",
+                                f"values have shape {tuple(values.shape)}, expected {shape} for layout {layout}")
         call("hcg_gather_channels", self.n, self.nchan, _ptr(self.plain.buf.perm), _ptr(values), self.dcode, layout,
              _ptr(self.values_sorted), _ptr(self.nonfinite), _stream_handle(stream))
 
@@ -257,6 +313,9 @@ class CubeGridder:
         plan: precompute the tile plan in a workspace (hcg_grid_mfma_ex)."""
         torch = _torch()
         b = self.plain.buf
+        m = self.side * self.side
+        check_tensor(xwcs, "grid", "xwcs", "f64", m, self.plain.device)
+        check_tensor(ywcs, "grid", "ywcs", "f64", m, self.plain.device)
         shp = capi.MfmaShape(*(shape if shape is not None else (0, 0, 0)), 0)
         ws, nb = None, 0
         if plan:
@@ -273,6 +332,7 @@ class CubeGridder:
     def gather_records(self, lon, lat, stream=None):
         """Sorted sample records and coordinates (the value slot stays 0)."""
         b = self.plain.buf
+        self.plain._check_coords("gather_records", lon, lat)
         call("hcg_gather_samples", self.n, _ptr(b.perm), _ptr(lon), _ptr(lat), None, capi.DTYPE_F64, None,
              _ptr(b.records), _ptr(b.lon_s), _ptr(b.lat_s), None, None, _stream_handle(stream))
 
