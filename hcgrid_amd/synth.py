@@ -30,11 +30,12 @@ class Target:
     crval1: float
     crval2: float
     crpix: float         # CRPIX1 = CRPIX2 (the reference generator's N/2)
+    proj: str = "SIN"    # zenithal projection: SIN, TAN, ARC, ZEA or STG
 
     def header_cards(self) -> dict:
         return {
             "NAXIS": 3, "NAXIS1": self.side, "NAXIS2": self.side, "NAXIS3": 1,
-            "CTYPE1": "RA---SIN", "CTYPE2": "DEC--SIN", "CUNIT1": "deg", "CUNIT2": "deg",
+            "CTYPE1": f"RA---{self.proj}", "CTYPE2": f"DEC--{self.proj}", "CUNIT1": "deg", "CUNIT2": "deg",
             "CDELT1": -self.cdelt, "CDELT2": self.cdelt, "CRPIX1": self.crpix,
             "CRPIX2": self.crpix, "CRVAL1": self.crval1, "CRVAL2": self.crval2,
         }
@@ -128,6 +129,19 @@ class CubeWorkload:
         return int(self.lon.size)
 
 
+def cube_coords(n_lines: int = 500, n_per_line: int = 4000, dec0: float = 25.01, ddec: float = 0.02,
+                ra0: float = 175.00125, dra: float = 0.0025, seed: int = 2):
+    """The sample coordinates of raster_cube (same arguments, same bytes),
+    without the channel values; also returns the generator, positioned where
+    raster_cube continues with the sources."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    j = np.arange(n_per_line, dtype=np.float64)
+    k = np.arange(n_lines, dtype=np.float64)
+    lat = np.repeat(dec0 + ddec * k, n_per_line)
+    lon = np.tile(ra0 + dra * j, n_lines) + rng.uniform(-dra / 6.4, dra / 6.4, size=n_lines * n_per_line)
+    return lon, lat, rng
+
+
 def raster_cube(n_lines: int = 500, n_per_line: int = 4000, dec0: float = 25.01, ddec: float = 0.02,
                 ra0: float = 175.00125, dra: float = 0.0025, nchan: int = 1024, side: int = 1024,
                 n_src: int = 20, seed: int = 2, device: str = "cuda", dtype: str = "bf16") -> CubeWorkload:
@@ -138,11 +152,8 @@ def raster_cube(n_lines: int = 500, n_per_line: int = 4000, dec0: float = 25.01,
     Values are generated on the device (torch, seeded) in channel-major order."""
     import torch
 
-    rng = np.random.Generator(np.random.PCG64(seed))
+    lon, lat, rng = cube_coords(n_lines, n_per_line, dec0, ddec, ra0, dra, seed)
     k = np.arange(n_lines, dtype=np.float64)
-    j = np.arange(n_per_line, dtype=np.float64)
-    lat = np.repeat(dec0 + ddec * k, n_per_line)
-    lon = np.tile(ra0 + dra * j, n_lines) + rng.uniform(-dra / 6.4, dra / 6.4, size=n_lines * n_per_line)
     n = lon.size
     tdt = torch.bfloat16 if dtype == "bf16" else torch.float16
     gen = torch.Generator(device=device)

@@ -79,6 +79,10 @@ def lib():
         L.orc_pix2ring_many.argtypes = [P(OrcHpx), C.c_int64, vp, vp]
         L.orc_ring_info_many.argtypes = [P(OrcHpx), C.c_int64, vp, vp, vp]
         L.orc_sample_keys.argtypes = [P(OrcHpx), C.c_int64, vp, vp, vp]
+        L.orc_sample_keys_libm_sensitive.restype = C.c_int64
+        L.orc_sample_keys_libm_sensitive.argtypes = [P(OrcHpx), C.c_int64, vp, vp, C.c_int, vp]
+        L.orc_great_circle_bearing.restype = C.c_double
+        L.orc_great_circle_bearing.argtypes = [C.c_double] * 4
         L.orc_search_last_less.restype = C.c_uint32
         L.orc_search_last_less.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_uint64]
         L.orc_presort.restype = None
@@ -165,6 +169,16 @@ def sample_keys(h: OrcHpx, lon_deg, lat_deg) -> np.ndarray:
     out = np.empty(lon.shape, dtype=np.uint64)
     lib().orc_sample_keys(C.byref(h), lon.size, _p(lon), _p(lat), _p(out))
     return out
+
+
+def keys_libm_sensitive(h: OrcHpx, lon_deg, lat_deg, ulps: int = 3):
+    """(count, flags u8[n]): samples whose key differs for some cos/sin
+    result within +-ulps of this libm's (the only library calls of ang2pix)."""
+    lon = np.ascontiguousarray(lon_deg, dtype=np.float64)
+    lat = np.ascontiguousarray(lat_deg, dtype=np.float64)
+    flags = np.zeros(lon.shape, dtype=np.uint8)
+    cnt = lib().orc_sample_keys_libm_sensitive(C.byref(h), lon.size, _p(lon), _p(lat), int(ulps), _p(flags))
+    return int(cnt), flags
 
 
 def presort(h: OrcHpx, lon_deg, lat_deg):

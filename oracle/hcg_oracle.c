@@ -244,6 +244,50 @@ void orc_sample_keys(const orc_hpx *h, int64_t n, const double *lon, const doubl
     }
 }
 
+/* Key sensitivity to the math library.  orc_ang2pix's only library calls
+ * are cos(theta) and, within 0.01 rad of a pole, sin(theta)
+ * (HCGrid/healpix.cu:208-224); everything else is IEEE-exact arithmetic.
+ * For each sample this re-evaluates the key with every double within
+ * +-ulps of this libm's cos (and sin) and flags the sample when any of
+ * them gives another pixel.  A sample that is not flagged has the same key
+ * under every cos/sin whose results lie within `ulps` of these -- the
+ * reference's glibc or CUDA libm, or the device's -- so an unflagged
+ * workload's keys are the reference's whatever libm computed them.
+ * Returns the number of flagged samples; flags (may be NULL) gets 0/1. */
+static double orc_step_ulps(double x, int k) {
+    while (k > 0) { x = nextafter(x, INFINITY); --k; }
+    while (k < 0) { x = nextafter(x, -INFINITY); ++k; }
+    return x;
+}
+
+int64_t orc_sample_keys_libm_sensitive(const orc_hpx *h, int64_t n, const double *lon, const double *lat,
+                                       int ulps, uint8_t *flags) {
+    int64_t count = 0;
+#pragma omp parallel for schedule(static) reduction(+ : count)
+    for (int64_t i = 0; i < n; ++i) {
+        double theta = ORC_HALFPI - ORC_DEG2RAD * lat[i];
+        double phi = ORC_DEG2RAD * lon[i];
+        int flag = 0;
+        if (theta >= 0 && theta <= ORC_PI) {
+            int have_sin = (theta < 0.01) || (theta > ORC_PI - 0.01);
+            double z0 = cos(theta), s0 = have_sin ? sin(theta) : 0.;
+            uint64_t k0 = orc_loc2pix(h, z0, phi, s0, have_sin);
+            for (int a = -ulps; a <= ulps && !flag; ++a) {
+                double z = orc_step_ulps(z0, a);
+                if (have_sin) {
+                    for (int b = -ulps; b <= ulps && !flag; ++b)
+                        flag = orc_loc2pix(h, z, phi, orc_step_ulps(s0, b), 1) != k0;
+                } else {
+                    flag = orc_loc2pix(h, z, phi, 0., 0) != k0;
+                }
+            }
+        }
+        if (flags) flags[i] = (uint8_t)flag;
+        count += flag;
+    }
+    return count;
+}
+
 /* ------------------------------------------------------------------ */
 /* pre-sort and the ring lookup table: HCGrid/gridding.cu:70-147       */
 /* ------------------------------------------------------------------ */

@@ -78,6 +78,9 @@ void orc_ring_info_many(const orc_hpx *h, int64_t n, const uint64_t *ring,
 /* sample keys exactly as the pre-sort computes them from (lon, lat) in degrees */
 void orc_sample_keys(const orc_hpx *h, int64_t n, const double *lon_deg,
                      const double *lat_deg, uint64_t *keys);
+/* samples whose key changes for some cos (sin) within +-ulps of this libm's */
+int64_t orc_sample_keys_libm_sensitive(const orc_hpx *h, int64_t n, const double *lon_deg,
+                                       const double *lat_deg, int ulps, uint8_t *flags);
 
 /* ---- pre-sort + ring LUT (HCGrid/gridding.cu:70-147, 260-275) ---- */
 uint32_t orc_search_last_less(const uint64_t *v, uint32_t left, uint32_t right, uint64_t key);

@@ -7,9 +7,15 @@
 * grid        : the loop restatement (LUT + binary search + break test)
                 equals a pair-by-pair evaluation of the selection rule, bit
                 for bit; and where the reference's window holds the whole
-                disc (|dec| < 20 deg, first used ring far away) it equals a
+                disc (|dec| < 20 deg, or the polar whole-ring branch at
+                |dec| >= 70 deg, first used ring far away) it equals a
                 textbook long-double disc gridder.
-* kernels     : closed-form values of the three weight functions.
+* multi       : the multi-channel form equals the single-channel loop run
+                once per channel, bit for bit (the reference's channel loop).
+* bearing     : closed forms (north 0, east pi/2, south pi, west -pi/2) and
+                the (east, north)-frame vector form in long double.
+* kernels     : closed-form values of the three weight functions; GAUSS2D
+                along its axes through the bearing.
 * WCS         : closed forms of the SIN deprojection.
 """
 import ctypes as C
@@ -261,3 +267,131 @@ def test_wcs_sin_closed_forms():
     py = (np.arange(side * side) // side) + 1
     assert np.allclose(-fx, c * (px - 51.0), atol=1e-11)  # CDELT1 < 0
     assert np.allclose(fy, c * (py - 51.0), atol=1e-11)
+
+
+@pytest.mark.parametrize("dec,factor", [(30.0, 1), (72.0, 1), (84.0, 1), (30.0, 2), (30.0, 3), (-45.0, 2)])
+def test_compat_multi_equals_compat_per_channel(dec, factor):
+    # orc_grid_compat_multi (weights once, every channel) must be what the
+    # reference's channel-by-channel pipeline gives: orc_grid_compat run once
+    # per channel on that channel's values, bit for bit, weights included
+    sp = 0.0032 / max(math.cos(math.radians(dec)), 0.1)
+    wl = synth.raster_small(30, 120, 16, dec_c=dec, spacing_ra=sp, seed=47)
+    r, xw, yw = _setup(wl, factor=factor)
+    rng = np.random.default_rng(5)
+    nchan = 5
+    V = rng.normal(0.0, 1.0, (nchan, wl.n)).astype(np.float32)
+    V[0] = wl.values  # channel 0: the single-channel data itself
+    Vs = V[:, r["perm"]]
+    side = int(r["g"].side)
+    out, ws = O.grid_compat_multi(r["h"], r["g"], r["lon_s"], r["lat_s"], Vs, None, r["keys_sorted"],
+                                  r["firstring"], r["usedrings"], r["lut"], xw, yw, 0, side)
+    assert (ws > 0).sum() > 0
+    for ch in range(nchan):
+        dc, wc, hangs = O.grid_compat(r["h"], r["g"], r["lon_s"], r["lat_s"], Vs[ch].astype(np.float64), None,
+                                      r["keys_sorted"], r["firstring"], r["usedrings"], r["lut"], xw, yw)
+        assert hangs == 0
+        assert np.array_equal(out[ch], dc), f"channel {ch}"
+        assert np.array_equal(ws, wc)
+    # a band of rows is the same band of the whole map
+    b0, b1 = 5, 9
+    ob, wb = O.grid_compat_multi(r["h"], r["g"], r["lon_s"], r["lat_s"], Vs, None, r["keys_sorted"],
+                                 r["firstring"], r["usedrings"], r["lut"], xw, yw, b0, b1)
+    assert np.array_equal(ob, out[:, b0 * side:b1 * side]) and np.array_equal(wb, ws[b0 * side:b1 * side])
+
+
+def test_compat_multi_with_sample_weights():
+    wl = synth.raster_small(30, 120, 16, seed=49)
+    r, xw, yw = _setup(wl)
+    w = np.random.default_rng(2).uniform(0.2, 2.0, wl.n)
+    ws_s = w[r["perm"]]
+    dc, wc, _ = O.grid_compat(r["h"], r["g"], r["lon_s"], r["lat_s"], r["data_s"], ws_s, r["keys_sorted"],
+                              r["firstring"], r["usedrings"], r["lut"], xw, yw)
+    out, ws = O.grid_compat_multi(r["h"], r["g"], r["lon_s"], r["lat_s"],
+                                  r["data_s"].astype(np.float32)[None, :], ws_s, r["keys_sorted"], r["firstring"],
+                                  r["usedrings"], r["lut"], xw, yw, 0, 16)
+    assert np.array_equal(out[0], dc) and np.array_equal(ws, wc)
+
+
+def _bearing(l1, b1, l2, b2):
+    L = O.lib()
+    L.orc_great_circle_bearing.restype = C.c_double
+    L.orc_great_circle_bearing.argtypes = [C.c_double] * 4
+    return L.orc_great_circle_bearing(l1, b1, l2, b2)
+
+
+def test_great_circle_bearing_closed_forms():
+    # bearing of (l2, b2) seen from (l1, b1), radians east of north
+    r = math.radians
+    assert _bearing(r(40), r(10), r(40), r(11)) == 0.0                    # due north
+    assert math.isclose(_bearing(r(40), r(10), r(40), r(9)), math.pi, rel_tol=1e-15)  # due south
+    assert math.isclose(_bearing(r(0), 0.0, r(1), 0.0), math.pi / 2, rel_tol=1e-15)   # due east on the equator
+    assert math.isclose(_bearing(r(0), 0.0, r(-1), 0.0), -math.pi / 2, rel_tol=1e-15)  # due west
+    # from a point towards the north pole, from the south pole towards anywhere
+    assert abs(_bearing(r(123), r(-20), r(77), r(90))) < 1e-15
+    # the back azimuth on the equator: swapping the points turns east into west
+    assert math.isclose(_bearing(r(1), 0.0, r(0), 0.0), -math.pi / 2, rel_tol=1e-15)
+
+
+def test_great_circle_bearing_against_vector_form():
+    # an independent construction: the direction of q in the local
+    # (east, north) frame of p, atan2(q.E, q.N), evaluated in long double
+    rng = np.random.default_rng(3)
+    n = 2000
+    l1 = rng.uniform(0, 2 * np.pi, n)
+    b1 = rng.uniform(-1.5, 1.5, n)
+    d = rng.uniform(1e-4, 0.2, n)
+    l2 = l1 + rng.normal(0, 1, n) * d
+    b2 = np.clip(b1 + rng.normal(0, 1, n) * d, -1.55, 1.55)
+    ld = np.longdouble
+    L1, B1, L2, B2 = (a.astype(ld) for a in (l1, b1, l2, b2))
+    qx, qy, qz = np.cos(B2) * np.cos(L2), np.cos(B2) * np.sin(L2), np.sin(B2)
+    ex, ey = -np.sin(L1), np.cos(L1)
+    nx, ny, nz = -np.sin(B1) * np.cos(L1), -np.sin(B1) * np.sin(L1), np.cos(B1)
+    want = np.arctan2(qx * ex + qy * ey, qx * nx + qy * ny + qz * nz).astype(np.float64)
+    got = np.array([_bearing(*v) for v in zip(l1, b1, l2, b2)])
+    assert np.max(np.abs(got - want)) < 1e-9  # separations >= 1e-4 rad: ~1e-12 rad of FP64 rounding
+    # arguments swapped (the bearing of p seen from q) would fail it
+    swapped = np.array([_bearing(c, d_, a, b) for a, b, c, d_ in zip(l1, b1, l2, b2)])
+    assert np.max(np.abs(swapped - want)) > 1.0
+
+
+def test_gauss2d_weight_along_the_axes():
+    # GAUSS2D (gridding.cu:242-249) with position angle 0: a sample due north
+    # of the cell is weighted exp(-d^2 / (2 p0^2)), one due east on the
+    # equator exp(-d^2 / (2 p1^2)) -- the pair of calls the compat loop makes
+    L = O.lib()
+    p0, p1 = 0.03, 0.015
+    g2 = O.grid_params(O.GAUSS2D, [p0, p1, 0.0], 0.1, 0.001, 4)
+    r = math.radians
+    d = 0.02
+    b_n = _bearing(r(10), r(5), r(10), r(5 + d))
+    b_e = _bearing(r(10), 0.0, r(10 + d), 0.0)
+    assert math.isclose(L.orc_kernel_weight(C.byref(g2), d, b_n), math.exp(-d * d / (2 * p0 * p0)), rel_tol=1e-13)
+    assert math.isclose(L.orc_kernel_weight(C.byref(g2), d, b_e), math.exp(-d * d / (2 * p1 * p1)), rel_tol=1e-13)
+
+
+@pytest.mark.parametrize("dec", [75.0, -80.0, 72.5])
+def test_compat_polar_whole_rings_equal_textbook_disc(dec):
+    # With the window anchor at colatitude <= 20 deg (or >= 160 deg) the
+    # reference takes whole rings (gridding.cu:336-341), and its rings span
+    # +-disc = R + resolution in colatitude: every sample within R of a cell
+    # is a candidate.  With the first used ring owned by a far sample, the
+    # compat selection is then exactly the disc, which the long-double
+    # textbook gridder evaluates with no rings at all.
+    sp = 0.0032 / math.cos(math.radians(dec))
+    wl = synth.raster_small(40, 160, 20, dec_c=dec, spacing_ra=sp, seed=53)
+    lon = np.concatenate([[wl.lon[0]], wl.lon])
+    lat = np.concatenate([[dec + 3.0], wl.lat])  # north of the field: the first used ring
+    val = np.concatenate([[0.0], wl.values.astype(np.float64)])
+    t = wl.target
+    xw, yw = O.wcs_sin(t.side, t.side, t.side / 2.0, t.side / 2.0, -t.cdelt, t.cdelt, t.crval1, t.crval2)
+    theta = 90.0 - yw
+    assert np.all((theta <= 20.0) | (theta >= 160.0))  # every cell's window is the whole-ring branch
+    r = O.pipeline(lon, lat, val, xw, yw, t.side, 0.02, 0.06, 0.01)
+    assert r["hangs"] == 0
+    dc, wc = O.grid_disc_textbook(lon, lat, val, xw, yw, 0.06, 0.5 / 0.02 ** 2)
+    assert (wc > 0).sum() > 0.5 * wc.size
+    e = map_errors(normalise(r["datacube"], r["weightscube"]), normalise(dc, wc))
+    assert e["nan_match"]
+    assert e["max_rel"] < 1e-12 and e["cell_rel"] < 1e-9
+    assert np.allclose(r["weightscube"], wc, rtol=1e-12, atol=0)
