@@ -151,7 +151,10 @@ PROTOTYPES = {
     "hcg_batch_enqueue": (_int, [_vp, _vp]),
     "hcg_batch_info": (_int, [_vp, C.POINTER(BatchStats)]),
     "hcg_batch_nonfinite": (_int, [_vp, C.POINTER(_u32), _vp]),
+    "hcg_batch_timing": (_int, [_vp, _int]),
+    "hcg_batch_last_times": (_int, [_vp, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "hcg_batch_destroy": (_int, [_vp]),
+    "hcg_content_hash": (_int, [_vp, _sz, _vp, _vp]),
     "hcg_graph_capture_begin": (_int, [_vp]),
     "hcg_graph_capture_end": (_int, [_vp, C.POINTER(_vp)]),
     "hcg_graph_launch": (_int, [_vp, _vp]),

@@ -64,6 +64,9 @@ struct hcg_batch {
   std::vector<MfmaItem> host_items;
   std::vector<PrepRange> host_ranges;
   std::vector<PrepJob> host_prep;
+  // optional kernel timing (hcg_batch_timing): start, after the prep launch, after the gridding launches
+  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  bool timed = false;  // the events of the last eager enqueue were recorded
 };
 
 namespace {
@@ -494,21 +497,60 @@ extern "C" int hcg_batch_create_in(uint32_t njobs, const hcg_batch_job* jobs, ui
 extern "C" int hcg_batch_enqueue(hcg_batch* b, void* stream) {
   HCG_REQUIRE(b, "hcg_batch_enqueue: NULL batch");
   hipStream_t s = as_stream(stream);
+  // timing events only outside stream capture (a captured graph carries no event records)
+  bool timing = b->ev[0] != nullptr;
+  if (timing) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    HCG_CHECK_HIP(hipStreamIsCapturing(s, &cs));
+    timing = cs == hipStreamCaptureStatusNone;
+  }
+  if (timing) HCG_CHECK_HIP(hipEventRecord(b->ev[0], s));
   const uint32_t nc = 2 + b->njobs;
   k_fill_u32<<<ceil_div_u32(nc, 256), 256, 0, s>>>(b->d_counters, nc, 0u);
   HCG_CHECK_LAUNCH("k_fill_u32");
   // every job's channel gather and tile plan, one launch
   int rc = launch_batch_prep(b->d_ranges, b->nranges, b->prep_blocks, b->d_prep, b->d_jobs, s);
   if (rc != HCG_OK) return rc;
+  if (timing) HCG_CHECK_HIP(hipEventRecord(b->ev[1], s));
   // the dense jobs' launch (its own queue head after the job counters), then the rest
   if (b->nitems_dense) {
     rc = launch_mfma_batch(kDenseChunk, kDenseWaves, b->blocks_dense, b->d_jobs, b->d_items, b->nitems_dense,
                            b->d_counters + 1 + b->njobs, s);
     if (rc != HCG_OK) return rc;
   }
-  if (b->nitems == b->nitems_dense) return HCG_OK;
-  return launch_mfma_batch(b->chunk, b->waves, b->blocks, b->d_jobs, b->d_items + b->nitems_dense,
+  if (b->nitems != b->nitems_dense) {
+    rc = launch_mfma_batch(b->chunk, b->waves, b->blocks, b->d_jobs, b->d_items + b->nitems_dense,
                            b->nitems - b->nitems_dense, b->d_counters, s);
+    if (rc != HCG_OK) return rc;
+  }
+  if (timing) {
+    HCG_CHECK_HIP(hipEventRecord(b->ev[2], s));
+    b->timed = true;
+  }
+  return HCG_OK;
+}
+
+extern "C" int hcg_batch_timing(hcg_batch* b, int enable) {
+  HCG_REQUIRE(b, "hcg_batch_timing: NULL batch");
+  if (enable && !b->ev[0]) {
+    for (auto& e : b->ev) HCG_CHECK_HIP(hipEventCreate(&e));
+  } else if (!enable && b->ev[0]) {
+    for (auto& e : b->ev) {
+      HCG_CHECK_HIP(hipEventDestroy(e));
+      e = nullptr;
+    }
+  }
+  b->timed = false;
+  return HCG_OK;
+}
+
+extern "C" int hcg_batch_last_times(hcg_batch* b, float* prep_ms, float* grid_ms) {
+  HCG_REQUIRE(b && prep_ms && grid_ms, "hcg_batch_last_times: NULL argument");
+  HCG_REQUIRE(b->ev[0] && b->timed, "hcg_batch_last_times: no timed enqueue (hcg_batch_timing first)");
+  HCG_CHECK_HIP(hipEventSynchronize(b->ev[2]));
+  HCG_CHECK_HIP(hipEventElapsedTime(prep_ms, b->ev[0], b->ev[1]));
+  HCG_CHECK_HIP(hipEventElapsedTime(grid_ms, b->ev[1], b->ev[2]));
+  return HCG_OK;
 }
 
 extern "C" int hcg_batch_info(const hcg_batch* b, hcg_batch_stats* st) {
@@ -534,6 +576,8 @@ extern "C" int hcg_batch_nonfinite(const hcg_batch* b, uint32_t* counts, void* s
 
 extern "C" int hcg_batch_destroy(hcg_batch* b) {
   if (!b) return HCG_OK;
+  for (auto& e : b->ev)
+    if (e) HCG_CHECK_HIP(hipEventDestroy(e));
   if (b->owns && b->base) HCG_CHECK_HIP(hipFree(b->base));
   delete b;
   return HCG_OK;

@@ -209,15 +209,22 @@ int hcg_gather_channels(uint32_t n, uint32_t nchan, const uint32_t *perm, const 
  * datacube planes); wsum (FP32 [side*side], may be NULL) receives sum w~.
  * NaN where a cell has no weight.  Samples are the sorted records/keys of
  * hcg_presort/hcg_gather_samples; values_sorted from hcg_gather_channels.
- * One channel (nchan == 1) with GAUSS1D, no weights_sorted, a radius below
- * ~0.26 deg and side >= 512 (at least 4096 tiles of 16 x 4 cells) runs the
- * single-channel FP64 tile loop instead of the contraction (w~ = the FP64
- * weight itself; the launch shape is ignored): with one channel the weight
- * is all the work, and that loop is several times faster at it.  2..16
- * channels under the same conditions take that loop too, with one FP32 sum
- * per channel (weights rounded once to FP32; wsum is the FP64 weight sum):
- * on sparse maps it avoids the contraction's fixed cost per block.  Batch
- * jobs take the same routes, so a job's cube is the same alone or batched. */
+ * Routes (which weight precision applies):
+ *  - one channel (nchan == 1) with GAUSS1D, no weights_sorted and a radius
+ *    below ~0.26 deg, on a map of any side, runs the single-channel FP64
+ *    tile loop instead of the contraction (w~ = the unrounded FP64 weight;
+ *    the launch shape is ignored).  Maps of at least 4096 tiles of 16 x 4
+ *    cells (side >= 512) give each tile one wave; smaller maps split each
+ *    tile's rings over 4 partial streams, summed in stream order;
+ *  - 2..16 channels with GAUSS1D, no weights_sorted, a radius between
+ *    ~0.018 deg (where the candidate test on the unit vectors' dot product
+ *    is exact enough) and ~0.26 deg, on maps of at least 4096 tiles, run
+ *    that loop too, with one FP32 sum per channel (weights rounded once to
+ *    FP32; wsum is the FP64 weight sum);
+ *  - everything else runs the MFMA contraction (w~ = weight rounded to
+ *    dtype as hi + lo).
+ * Batch jobs take the same routes, so a job's cube is the same alone or
+ * batched. */
 int hcg_grid_mfma(const hcg_healpix *hp, const hcg_kernel *kern, uint32_t side, uint32_t n, uint32_t nchan,
                   int dtype, const hcg_sample *records, const uint32_t *keys_sorted,
                   const double *lon_sorted, const double *lat_sorted, const double *weights_sorted,
@@ -413,7 +420,28 @@ int hcg_batch_info(const hcg_batch *b, hcg_batch_stats *stats);
  * array of njobs; synchronises the stream). */
 int hcg_batch_nonfinite(const hcg_batch *b, uint32_t *counts, void *stream);
 
+/* Kernel timing of hcg_batch_enqueue, for measurement (no reference
+ * counterpart).  enable != 0 creates three HIP events in the batch; every
+ * later enqueue that is not being captured into a graph records them on its
+ * stream: before the counter fill, after the prep launch (channel gathers
+ * and tile plans), and after the persistent gridding launches.
+ * hcg_batch_last_times waits for the last such enqueue and returns the prep
+ * and gridding durations in milliseconds.  enable == 0 destroys the events. */
+int hcg_batch_timing(hcg_batch *b, int enable);
+int hcg_batch_last_times(hcg_batch *b, float *prep_ms, float *grid_ms);
+
 int hcg_batch_destroy(hcg_batch *b);
+
+/* 128-bit content hash of `bytes` bytes of device memory (8-byte aligned):
+ * out (device, 2 x uint64, 8-byte aligned) receives
+ *   h_k = mix64(bytes ^ B_(1-k)) + sum_i mix64(w_i ^ (i * A_k + B_k)),
+ * w_i the buffer's 8-byte little-endian words (the tail zero-padded),
+ * mix64 the splitmix64 finaliser, A_0 = 0x9e3779b97f4a7c15,
+ * B_0 = 0x243f6a8885a308d3, A_1 = 0xc2b2ae3d27d4eb4f, B_1 = 0x13198a2e03707344,
+ * all modulo 2^64.  Deterministic (the sum is order-free).  No reference
+ * counterpart: the served loop's key for resident geometries (samples
+ * sorted once, reused for repeat coordinates). */
+int hcg_content_hash(const void *data, size_t bytes, uint64_t *out, void *stream);
 
 /* Graph capture of any sequence of the entry points above on a created
  * stream (not the null stream), then replay.  No reference counterpart. */
