@@ -116,6 +116,7 @@ PROTOTYPES = {
     "hcg_host_pix2theta": (_dbl, [C.POINTER(Healpix), _u64]),
     "hcg_presort_workspace_size": (_sz, [_u32]),
     "hcg_presort": (_int, [C.POINTER(Healpix), _u32, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
+    "hcg_presort_noalloc": (_int, [C.POINTER(Healpix), _u32, _vp, _vp, _vp, _vp, _vp]),
     "hcg_gather_samples": (_int, [_u32, _vp, _vp, _vp, _vp, _int, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "hcg_ring_lut": (_int, [C.POINTER(Healpix), _u32, _vp, _vp, _vp, _vp, _vp, _vp]),
     "hcg_grid_plain": (_int, [C.POINTER(Healpix), C.POINTER(Kernel), _u32, _u32, _vp, _vp, _vp, _vp, _vp,

@@ -19,6 +19,8 @@
 // MAX_Y = 2^30), so 4 passes of 8 bits order them completely.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "common.hpp"
 #include "healpix.hpp"
 
@@ -267,47 +269,45 @@ struct SortWorkspace {
   uint32_t tiles;
 };
 
-size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+constexpr size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// Sort scratch for n keys: ping-pong keys and values, per-tile digit
+// counts, per-digit totals.  Every byte of it is written before it is read.
+constexpr size_t sort_ws_bytes(uint32_t n) {
+  return 2 * align256((size_t)n * 4) + align256((size_t)((n ? n : 1) + kSortTile - 1) / kSortTile * 256 * 4) +
+         align256(256 * 4);
+}
 
 size_t sort_ws_layout(uint32_t n, void* base, SortWorkspace* ws) {
   const uint32_t tiles = ceil_div_u32(n ? n : 1, kSortTile);
-  size_t off = 0;
-  const size_t kb = align256((size_t)n * 4), hb = align256((size_t)tiles * 256 * 4), db = align256(256 * 4);
+  const size_t kb = align256((size_t)n * 4), hb = align256((size_t)tiles * 256 * 4);
   if (ws) {
     char* p = static_cast<char*>(base);
-    ws->keys_b = reinterpret_cast<uint32_t*>(p + off);
-    ws->vals_b = reinterpret_cast<uint32_t*>(p + off + kb);
-    ws->hist = reinterpret_cast<uint32_t*>(p + off + 2 * kb);
-    ws->digit = reinterpret_cast<uint32_t*>(p + off + 2 * kb + hb);
+    ws->keys_b = reinterpret_cast<uint32_t*>(p);
+    ws->vals_b = reinterpret_cast<uint32_t*>(p + kb);
+    ws->hist = reinterpret_cast<uint32_t*>(p + 2 * kb);
+    ws->digit = reinterpret_cast<uint32_t*>(p + 2 * kb + hb);
     ws->tiles = tiles;
   }
-  off = 2 * kb + hb + db;
-  return off;
+  return sort_ws_bytes(n);
 }
-}  // namespace
 
-extern "C" size_t hcg_presort_workspace_size(uint32_t n) { return sort_ws_layout(n, nullptr, nullptr); }
-
-extern "C" int hcg_presort(const hcg_healpix* hp, uint32_t n, const double* lon_deg, const double* lat_deg,
-                           uint32_t* keys_sorted, uint32_t* perm, void* workspace, size_t workspace_bytes,
-                           void* stream) {
-  HCG_REQUIRE(hp != nullptr, "hcg_presort: hp is NULL");
-  HCG_REQUIRE(keys_sorted != nullptr && perm != nullptr, "hcg_presort: NULL output");
+// Argument checks shared by hcg_presort and hcg_presort_noalloc.
+int presort_args(const hcg_healpix* hp, uint32_t n, const double* lon_deg, const double* lat_deg,
+                 const uint32_t* keys_sorted, const uint32_t* perm, const char* who) {
+  if (hp == nullptr) return set_error(HCG_ERR_INVALID, std::string(who) + ": hp is NULL");
+  if (keys_sorted == nullptr || perm == nullptr) return set_error(HCG_ERR_INVALID, std::string(who) + ": NULL output");
   if (hp->nside == 0 || hp->nside > HCG_MAX_NSIDE || (hp->nside & (hp->nside - 1)))
-    return set_error(HCG_ERR_UNSUPPORTED, "hcg_presort: nside must be a power of two <= 8192");
-  HCG_REQUIRE(n < 0xFFFFFFFFu, "hcg_presort: n must be < 2^32-1");
-  hipStream_t s = as_stream(stream);
-  if (n == 0) {
-    k_set_sentinel<<<1, 64, 0, s>>>(keys_sorted, 0, (uint32_t)hp->npix);
-    HCG_CHECK_LAUNCH("k_set_sentinel");
-    return HCG_OK;
-  }
-  HCG_REQUIRE(lon_deg != nullptr && lat_deg != nullptr, "hcg_presort: NULL input");
-  if (workspace == nullptr || workspace_bytes < hcg_presort_workspace_size(n))
-    return set_error(HCG_ERR_WORKSPACE, "hcg_presort: workspace smaller than hcg_presort_workspace_size(n)");
-  SortWorkspace ws;
-  sort_ws_layout(n, workspace, &ws);
+    return set_error(HCG_ERR_UNSUPPORTED, std::string(who) + ": nside must be a power of two <= 8192");
+  if (n >= 0xFFFFFFFFu) return set_error(HCG_ERR_INVALID, std::string(who) + ": n must be < 2^32-1");
+  if (n && (lon_deg == nullptr || lat_deg == nullptr))
+    return set_error(HCG_ERR_INVALID, std::string(who) + ": NULL input");
+  return HCG_OK;
+}
 
+// Keys, the four radix passes and the sentinel, on workspace `ws` (n > 0).
+int presort_enqueue(const hcg_healpix* hp, uint32_t n, const double* lon_deg, const double* lat_deg,
+                    uint32_t* keys_sorted, uint32_t* perm, const SortWorkspace& ws, hipStream_t s) {
   const uint32_t blocks = std::min<uint32_t>(ceil_div_u32(n, 256), 8192);
   k_keys<<<blocks, 256, 0, s>>>(*hp, n, lon_deg, lat_deg, keys_sorted, perm);
   HCG_CHECK_LAUNCH("k_keys");
@@ -333,6 +333,77 @@ extern "C" int hcg_presort(const hcg_healpix* hp, uint32_t n, const double* lon_
   k_set_sentinel<<<1, 64, 0, s>>>(keys_sorted, n, (uint32_t)hp->npix);
   HCG_CHECK_LAUNCH("k_set_sentinel");
   return HCG_OK;
+}
+
+// The no-allocation variant's scratch: one sort workspace for up to
+// HCG_PRESORT_NOALLOC_MAX_N keys in a file-scope device array.  In host code
+// `scratch` names a host-side stand-in, not a device address: its device
+// address comes from hipGetSymbolAddress, once per device (each device
+// holds its own copy of the array, at its own address).
+constexpr size_t kNoAllocScratchBytes = sort_ws_bytes(HCG_PRESORT_NOALLOC_MAX_N);
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The current device's address of `scratch` (resolved on the first call on
+// that device).
+int scratch_address(void** out) {
+  int dev = 0;
+  HCG_CHECK_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return set_error(HCG_ERR_UNSUPPORTED, "hcg_presort_noalloc: device index");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    HCG_CHECK_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return set_error(HCG_ERR_HIP, "hcg_presort_noalloc: no device address for the scratch array");
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return HCG_OK;
+}
+}  // namespace
+
+extern "C" size_t hcg_presort_workspace_size(uint32_t n) { return sort_ws_layout(n, nullptr, nullptr); }
+
+extern "C" int hcg_presort(const hcg_healpix* hp, uint32_t n, const double* lon_deg, const double* lat_deg,
+                           uint32_t* keys_sorted, uint32_t* perm, void* workspace, size_t workspace_bytes,
+                           void* stream) {
+  int rc = presort_args(hp, n, lon_deg, lat_deg, keys_sorted, perm, "hcg_presort");
+  if (rc != HCG_OK) return rc;
+  hipStream_t s = as_stream(stream);
+  if (n == 0) {
+    k_set_sentinel<<<1, 64, 0, s>>>(keys_sorted, 0, (uint32_t)hp->npix);
+    HCG_CHECK_LAUNCH("k_set_sentinel");
+    return HCG_OK;
+  }
+  if (workspace == nullptr || workspace_bytes < hcg_presort_workspace_size(n))
+    return set_error(HCG_ERR_WORKSPACE, "hcg_presort: workspace smaller than hcg_presort_workspace_size(n)");
+  SortWorkspace ws;
+  sort_ws_layout(n, workspace, &ws);
+  return presort_enqueue(hp, n, lon_deg, lat_deg, keys_sorted, perm, ws, s);
+}
+
+extern "C" int hcg_presort_noalloc(const hcg_healpix* hp, uint32_t n, const double* lon_deg, const double* lat_deg,
+                                   uint32_t* keys_sorted, uint32_t* perm, void* stream) {
+  int rc = presort_args(hp, n, lon_deg, lat_deg, keys_sorted, perm, "hcg_presort_noalloc");
+  if (rc != HCG_OK) return rc;
+  if (n > HCG_PRESORT_NOALLOC_MAX_N)
+    return set_error(HCG_ERR_UNSUPPORTED, "hcg_presort_noalloc: n exceeds HCG_PRESORT_NOALLOC_MAX_N (8388608); "
+                                          "use hcg_presort with a workspace");
+  hipStream_t s = as_stream(stream);
+  if (n == 0) {
+    k_set_sentinel<<<1, 64, 0, s>>>(keys_sorted, 0, (uint32_t)hp->npix);
+    HCG_CHECK_LAUNCH("k_set_sentinel");
+    return HCG_OK;
+  }
+  void* base = nullptr;
+  rc = scratch_address(&base);
+  if (rc != HCG_OK) return rc;
+  SortWorkspace ws;
+  sort_ws_layout(n, base, &ws);
+  return presort_enqueue(hp, n, lon_deg, lat_deg, keys_sorted, perm, ws, s);
 }
 
 extern "C" int hcg_gather_samples(uint32_t n, const uint32_t* perm, const double* lon_deg, const double* lat_deg,

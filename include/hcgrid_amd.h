@@ -140,6 +140,23 @@ int hcg_presort(const hcg_healpix *hp, uint32_t n, const double *lon_deg, const 
                 uint32_t *keys_sorted, uint32_t *perm, void *workspace, size_t workspace_bytes,
                 void *stream);
 
+/* hcg_presort without a workspace argument and without allocating (no
+ * hipMalloc, hipFree or allocating library call): no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid it).  The
+ * sort's scratch is a static device array of the library (66 MiB per GPU,
+ * held until the process ends), so n <= HCG_PRESORT_NOALLOC_MAX_N
+ * (2^23 = 8388608 samples); larger n is rejected with
+ * HCG_ERR_UNSUPPORTED.  Results equal hcg_presort's bit for bit.  Every
+ * call on one GPU uses that GPU's one array: calls that overlap on a GPU
+ * (other streams or host threads) race and must be serialised by the
+ * caller.  Nothing in the array needs clearing between calls (every byte
+ * is written before it is read).  The first call on a device resolves the
+ * array's device address (hipGetSymbolAddress), which enqueues no work. */
+#define HCG_PRESORT_NOALLOC_MAX_N 8388608u
+int hcg_presort_noalloc(const hcg_healpix *hp, uint32_t n, const double *lon_deg, const double *lat_deg,
+                        uint32_t *keys_sorted, uint32_t *perm, void *stream);
+
 /* The three permutation gathers of init_input_with_cpu (HCGrid/gridding.cu:101-114),
  * fused, plus the sample records the gridding kernel reads.  data has
  * data_dtype (HCG_DTYPE_F64 or HCG_DTYPE_F32).  weights may be NULL (all 1,
