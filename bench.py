@@ -342,8 +342,8 @@ def bench_cube(args, log):
     updates = wl.n * wl.nchan
     log(f"cube: {ms_step:.3f} ms/step, k_grid_mfma avg {ms_grid:.3f} ms")
 
-    # device results of the checked band: rows 510-513, every 32nd channel
-    r0, r1 = t.side // 2 - 2, t.side // 2 + 2
+    # device results of the checked band: rows 504-519, every 32nd channel
+    r0, r1 = t.side // 2 - 8, t.side // 2 + 8
     chans = list(range(0, wl.nchan, 32))
     band = cg.cube.view(wl.nchan, t.side, t.side)[chans][:, r0:r1, :].cpu().numpy().reshape(-1)
     dev_perm = cg.plain.perm()
@@ -438,7 +438,7 @@ def bench_cube(args, log):
 
 def cpu_cube(wl, xw, yw, r0, r1, chans, vsub, band, dev_perm, log):
     """Oracle on the cube workload: the full pre-sort, gathers and LUT, then
-    the multi-channel compat loop over `rows` map rows twice, with no
+    the multi-channel compat loop over a band of map rows twice, with no
     channels (selection and FP64 weights alone) and with the checked
     channel subset.  The weight cost per row and the cost per channel and
     row give the whole cube's time, scaled linearly in rows and channels.
@@ -471,7 +471,6 @@ def cpu_cube(wl, xw, yw, r0, r1, chans, vsub, band, dev_perm, log):
     total = (t1 - t0) + gather_full + (t3 - t2) + side * (per_row_w + per_row_ch * wl.nchan)
     log(f"cpu cube: sort {t1 - t0:.2f}s, band of {rows} rows: weights {t4 - t3:.2f}s, +{len(chans)} channels "
         f"{(t5 - t4) - (t4 - t3):.2f}s -> whole cube {total:.1f}s scaled")
-    o = out / ws[None, :] if ws.size else out
     with np.errstate(invalid="ignore", divide="ignore"):
         o = out / ws[None, :]
     err = map_error(band, o.reshape(-1))
@@ -653,7 +652,7 @@ def bench_batch(args, log):
     }
     if args.cpu_baseline:
         out["cpu_baseline"], out["error"] = cpu_batch(jobs, checked, log)
-    return out
+    return out, tuner
 
 
 def cpu_batch(jobs, checked, log):
@@ -722,21 +721,26 @@ def cpu_batch(jobs, checked, log):
 
 # -- served ---------------------------------------------------------------------------
 
-def bench_served(args, log):
+def bench_served(args, log, tuner=None):
     """Served gridding: Poisson arrivals at 20 jobs/s drawn from the batched
     mix (30% repeats of one of the last 32 distinct maps with new channel
     data), HBM admission budget 200 GB, two slots; p50/p99 of completion -
     arrival and throughput over the run.  The load generator makes each
     request's coordinates and values on the device before submitting it
-    (no host-to-device transfer in the latency); results stay in HBM."""
+    (no host-to-device transfer in the latency); results stay in HBM.
+    tuner: the batched leg's autotuner (block shape per job class), when
+    that leg ran; the persistent launch runs at its tuned waves/chunk."""
     from hcgrid_amd import serve
 
+    waves, chunk = 4, 256
+    if tuner is not None and "batch" in tuner.launch:
+        waves, chunk = (int(x) for x in tuner.launch["batch"])
     warm = serve.request_stream(12, rate=20.0, seed=99)
-    srv = serve.GriddingServer(budget_bytes=200e9, slots=2)
+    srv = serve.GriddingServer(budget_bytes=200e9, slots=2, waves=waves, chunk=chunk, tuner=tuner)
     serve.run_load(srv, warm)
     srv.close()
     stream = serve.request_stream(args.served_jobs, rate=20.0, seed=4, repeat_frac=0.3, recent=32)
-    srv = serve.GriddingServer(budget_bytes=200e9, slots=2)
+    srv = serve.GriddingServer(budget_bytes=200e9, slots=2, waves=waves, chunk=chunk, tuner=tuner)
     t0 = time.perf_counter()
     done = serve.run_load(srv, stream,
                           progress=lambda k, n: log(f"served: {k}/{n} requests done, "
@@ -753,6 +757,8 @@ def bench_served(args, log):
                      "maps with new channel data, HBM budget 200 GB, 2 slots; the load generator makes each "
                      "request's inputs in HBM before submitting it, cubes left in HBM"),
         "unit": UNIT,
+        "launch": {"waves": waves, "chunk": chunk,
+                   "tile_cols": "autotuned per job class" if tuner is not None else "default"},
         "server": stats,
     })
     return summ
@@ -855,11 +861,12 @@ def main():
     if args.cube:
         result["multichannel"] = bench_cube(args, log)
         torch.cuda.empty_cache()
+    tuner = None
     if args.batch:
-        result["batched"] = bench_batch(args, log)
+        result["batched"], tuner = bench_batch(args, log)
         torch.cuda.empty_cache()
     if args.served:
-        result["served"] = bench_served(args, log)
+        result["served"] = bench_served(args, log, tuner)
         torch.cuda.empty_cache()
     print(json.dumps(result), flush=True)
 

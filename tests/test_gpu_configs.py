@@ -3,7 +3,8 @@
 * 20M-sample raster onto a 2048 x 2048 SIN map (the single-channel drop-in
   at scale): HEALPix keys, sort order, firstring/usedrings and the ring LUT
   bit-exact over all 20M samples; map and weight sums within 1e-6 of the
-  oracle on three bands of rows (first 16, 128 in the middle, last 16).
+  oracle on three bands of rows (16 across each edge of the scanned field,
+  128 in the middle).
 * 64 uneven batched jobs (N log-uniform in [5e4, 5e6], sides 128..2048,
   1..512 BF16/FP16 channels, sigma 0.015/0.02/0.03 deg, Dec -60..80 deg) in
   one captured graph (every geometry build + the persistent launch): every
@@ -63,7 +64,10 @@ def test_raster_20m_keys_lut_and_map_bands(gpu):
 
     lon_s, lat_s = wl.lon[perm], wl.lat[perm]
     data_s = wl.values.astype(np.float64)[perm]
-    for r0, r1 in ((0, 16), (960, 1088), (side - 16, side)):
+    # the map (20.48 deg) is wider than the scanned field (20 deg): rows 16-31
+    # and 2016-2031 straddle its lower and upper edges (empty rows, partly
+    # covered rows), 960-1087 lie in the middle
+    for r0, r1 in ((16, 32), (960, 1088), (side - 32, side - 16)):
         print(f"case: raster_20m rows {r0}-{r1 - 1} against the oracle", flush=True)
         dc, wc = oracle.grid_compat_rows(h, gp, lon_s, lat_s, data_s, None, keys, fr, used, lut, xw, yw, r0, r1)
         sl = slice(r0 * side, r1 * side)
