@@ -252,6 +252,41 @@ __device__ inline double gauss_weight_series(const GridParams& P, double c2, con
   return exp_scaled_neg_fold(c2 * t, s_exp);
 }
 
+// A loop-invariant double moved into a vector register pair.  A VOP3
+// instruction on gfx950 reads at most one scalar operand, so an FMA whose
+// two constant operands both sit in scalar registers costs an extra move
+// (or a mul + add); with one of them in vector registers it is one v_fma_f64.
+__device__ inline double in_vgprs(double x) {
+  double v;
+  asm("v_mov_b64 %0, %1" : "=v"(v) : "s"(x));
+  return v;
+}
+
+// Degree-3 coefficient of exp(u ln2/64) - 1 = u (e1 + u (e2 + u e3)).
+constexpr double kExpE1 = 0x1.62e42fefa39efp-7;  // ln2 / 64
+constexpr double kExpE2 = kExpE1 * kExpE1 / 2.0;
+constexpr double kExpE3 = kExpE2 * kExpE1 / 3.0;
+
+// The GAUSS1D weight of a selected pair for the p.q tile loop
+// (gauss_tiny_radius && dot_distance_ok: c2 < 2e-5): the exponent's series
+// to its second term, y = c2 (k0 + c2 k1) in units of ln2/64 (the c2^2
+// term is below 5e-12 of the exponent there), and exp by the 64-entry table
+// (Tang 1989) with the degree-3 polynomial of exp(r) - 1, |r| <= ln2/128
+// (truncation r^4/24 < 3.6e-11 relative).  The weight is within 4e-11
+// relative of exp(-p0 d^2).  k1v and e3v hold k1 and kExpE3 in vector
+// registers (in_vgprs): every step is one v_fma_f64.
+__device__ inline double gauss_weight_lean(double c2, double k0, double k1v, double e3v, const double* s_exp) {
+  const double y = c2 * fma(c2, k1v, k0);
+  const double n = rint(y);
+  const int ni = (int)n;
+  const double u = y - n;
+  double q = fma_vvs(u, e3v, kExpE2);  // three-operand v_fma_f64: no copy of a constant register
+  q = fma_vvs(u, q, kExpE1);
+  const double p = u * q;
+  const double t = s_exp[ni & 63];
+  return ldexp(fma(t, p, t), ni >> 6);
+}
+
 // GAUSS2D / TAPERED_SINC weights (HCGrid/gridding.cu:242-256).
 template <int KT>
 [[maybe_unused]] static __device__ __noinline__ double weight_general(double p0, double p1, double p2, const double* xwcs,

@@ -312,6 +312,10 @@ __device__ __forceinline__ void grid_tile(const GridParams P, const GridBuffers 
       (void)je;
       const double dot_lo = fma(-0.5, P.c_hi, 1.0);  // c2 < c_hi  <=>  p.q > 1 - c_hi / 2
       (void)dot_lo;
+      // WMODE 3: the lean weight's constants in vector registers (hoisted out of the loops)
+      const double k1v = WMODE == 3 ? in_vgprs(P.k1) : 0.0, e3v = WMODE == 3 ? in_vgprs(kExpE3) : 0.0;
+      (void)k1v;
+      (void)e3v;
       for (uint32_t t = 0; t < len[0]; ++t) {
         const uint32_t j = lo[0] + t;
         const double qx = s_x[j], qy = s_y[j], qz = s_z[j], qv = NC > 0 ? 0.0 : s_v[j];
@@ -339,7 +343,9 @@ __device__ __forceinline__ void grid_tile(const GridParams P, const GridBuffers 
 #endif
             if (c2 < P.c_lo || radius_test_ref(B.xwcs, B.ywcs, B.lon_s, B.lat_s, cell[k], cb + j, P.sphere_radius)) {
               double wt;
-              if constexpr (KT == HCG_GAUSS1D && WMODE > 0)
+              if constexpr (KT == HCG_GAUSS1D && WMODE == 3 && HCG_GRID_CELLS_PER_LANE == 1 && !HCG_GRID_LANE_LOOP)
+                wt = gauss_weight_lean(c2, P.k0, k1v, e3v, s_exp);
+              else if constexpr (KT == HCG_GAUSS1D && WMODE > 0)
                 wt = gauss_weight_series<WMODE == 3 ? 2 : WMODE>(P, c2, s_exp);
               else if constexpr (KT == HCG_GAUSS1D)
                 wt = HCG_GRID_TABLE_EXP ? exp_scaled_neg(gauss_exponent_scaled(P, c2), s_exp)
