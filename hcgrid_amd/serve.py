@@ -26,6 +26,11 @@ A request that cannot be served -- invalid arguments, larger than the whole
 budget, or a failure while its work is set up or launched -- completes with
 `error` set and `cube` None; the other requests of its round go on.
 
+When a request completes (served or failed) the server drops its
+references to the request's inputs (lon, lat, values and the ready event
+become None), so a long run holds only the caller's own references and,
+with keep_results, the cubes.
+
 The reference has no server (HCGrid/HCGrid.cpp:38-197 grids one map per
 process run); this is the request loop around the batched path, and
 HCGrid+'s overlap of host work with device work (HCGrid+/gridding-omp.cu:
@@ -44,6 +49,7 @@ import math
 import queue
 import threading
 import time
+import traceback
 
 import numpy as np
 
@@ -75,15 +81,21 @@ class Request:
     cube: object = None    # float32 [nchan][side*side] on the device (kept with keep_results)
     wsum: object = None
     error: object = None
+    trace: str = ""        # the failure's formatted traceback (the exception keeps none)
     _pinned: object = None  # the cache entry this request holds a reference on
     _held: object = None    # device inputs kept alive until the round completes
+    _shape: object = None   # (n, nchan), recorded at submit: the inputs are released on completion
 
     @property
     def n(self) -> int:
+        if self._shape is not None:
+            return self._shape[0]
         return int(len(self.lon)) if isinstance(self.lon, np.ndarray) else int(self.lon.numel())
 
     @property
     def nchan(self) -> int:
+        if self._shape is not None:
+            return self._shape[1]
         return int(self.values.shape[0])
 
     @property
@@ -290,6 +302,7 @@ class GriddingServer:
         self.finished: list[Request] = []
         self.failed: list[Request] = []
         self.stats = collections.Counter()
+        self._abort = None
 
     # -- client side ---------------------------------------------------------
     def submit(self, req: Request):
@@ -300,12 +313,18 @@ class GriddingServer:
             req.arrival = time.perf_counter()
         try:
             validate(req)
+            req._shape = (req.n, req.nchan)
             t = req.target
             coords = ("id", req.geo_id) if req.geo_id is not None else self.be.content_key(req)
             req.geo_key = (coords, req.n, t.proj, t.side, t.cdelt, t.crval1, t.crval2, t.crpix, float(req.sigma))
         except Exception as exc:  # reported on the request, never raised into the loop
-            req.error = exc
+            self._set_error(req, exc)
         self.inbox.put(req)
+
+    def abort(self, exc: BaseException):
+        """Makes serve() raise (from any thread): e.g. the producer of the
+        requests failed and the remaining ones will never arrive."""
+        self._abort = exc
 
     # -- footprint and the geometry cache --------------------------------------
     def _evict(self, need: int) -> bool:
@@ -321,15 +340,38 @@ class GriddingServer:
                 self.stats["evictions"] += 1
         return self.used + need <= self.budget
 
-    def _fail(self, r: Request, exc: Exception):
+    @staticmethod
+    def _release_inputs(r: Request):
+        if r._shape is None:
+            try:
+                r._shape = (r.n, r.nchan)
+            except Exception:  # malformed inputs (failed validation)
+                r._shape = (0, 0)
+        r.lon = r.lat = r.values = r.ready = None
+        r._held = None
+
+    @staticmethod
+    def _set_error(r: Request, exc: BaseException):
+        # the traceback as text: its frames would keep the round's other
+        # requests' data (and this request's) alive as long as the error
+        r.trace = "".join(traceback.format_exception(type(exc), exc, exc.__traceback__))
+        e = exc
+        for _ in range(8):  # the exception and its causes
+            if e is None:
+                break
+            e.__traceback__ = None
+            e = e.__cause__ or e.__context__
         r.error = exc
+
+    def _fail(self, r: Request, exc: Exception):
+        self._set_error(r, exc)
         r.done = time.perf_counter()
         self.used -= r.footprint
         r.footprint = 0
         if r._pinned is not None:
             r._pinned.refs -= 1
             r._pinned = None
-        r._held = None
+        self._release_inputs(r)
         self.failed.append(r)
         self.stats["failed"] += 1
 
@@ -345,6 +387,7 @@ class GriddingServer:
             if r.error is not None:  # failed at submit
                 self.pending.popleft()
                 r.done = time.perf_counter()
+                self._release_inputs(r)
                 self.failed.append(r)
                 self.stats["failed"] += 1
                 continue
@@ -359,8 +402,12 @@ class GriddingServer:
                 if entry is not None:
                     entry.refs -= 1
                 in_flight = any(x is not None for x in self.rounds)
-                if not take and not in_flight and not any(e.refs for e in self.geoms.values()):
+                if not take and not in_flight:
                     # cannot fit even on an idle device with every geometry evicted
+                    # (with no round in flight no request holds a geometry)
+                    pinned = sum(e.refs for e in self.geoms.values())
+                    if pinned:
+                        raise RuntimeError(f"served loop: {pinned} geometry references held with no round in flight")
                     self.pending.popleft()
                     self._fail(r, capi.HcgError(capi.HCG_ERR_NOMEM, f"request {r.rid}",
                                                 f"needs {need} bytes, budget {self.budget}"))
@@ -466,7 +513,7 @@ class GriddingServer:
                 r._pinned = None
                 if self.keep:
                     r.cube, r.wsum = r._held[3], r._held[4]
-                r._held = None
+                self._release_inputs(r)
                 self.finished.append(r)
             for b in batches:
                 b.close()
@@ -477,13 +524,17 @@ class GriddingServer:
     # -- main loop -----------------------------------------------------------------
     def serve(self, n_total: int, timeout_s: float = 3600.0, progress=None, every: int = 500):
         """Runs until n_total requests have completed (served or failed);
-        progress(done, total) is called each time `every` more finish."""
+        progress(done, total) is called each time `every` more finish.
+        Raises if abort() was called or on the timeout."""
         t_end = time.perf_counter() + timeout_s
         next_report = every
         while len(self.finished) + len(self.failed) < n_total:
             if progress is not None and len(self.finished) >= next_report:
                 progress(len(self.finished), n_total)
                 next_report += every
+            if self._abort is not None:
+                raise RuntimeError(f"served loop aborted after {len(self.finished)} of {n_total} requests: "
+                                   f"{self._abort!r}") from self._abort
             if time.perf_counter() > t_end:
                 raise TimeoutError(f"served {len(self.finished)} of {n_total} requests before the timeout")
             busy = False
@@ -581,25 +632,29 @@ def run_load(server: GriddingServer, stream: list, time_scale: float = 1.0, prog
     import torch
     t0 = time.perf_counter()
     errors = []
+    stop = threading.Event()
 
     def client():
         try:
             for off, spec in stream:
                 wait = t0 + off * time_scale - time.perf_counter()
-                if wait > 0:
-                    time.sleep(wait)
+                if stop.wait(wait) if wait > 0 else stop.is_set():
+                    return
                 r = spec.make()
                 r.ready.synchronize()
                 r.arrival = time.perf_counter()
                 server.submit(r)
-        except Exception as exc:  # surfaces in the loop's thread below
+                del r  # the server holds the request's inputs until it completes
+        except Exception as exc:  # stops the loop, raised in its thread
             errors.append(exc)
+            server.abort(exc)
 
     th = threading.Thread(target=client, daemon=True)
     th.start()
     try:
         done = server.serve(len(stream), progress=progress)
     finally:
+        stop.set()  # the loop ended (or raised): the client stops producing
         th.join()
         torch.cuda.synchronize()
     if errors:

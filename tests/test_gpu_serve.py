@@ -115,8 +115,11 @@ def test_served_invalid_request_fails_alone(gpu):
             assert np.array_equal(got[s.rid], _alone(s)[0], equal_nan=True)
 
 
-def _oracle_check(r, cube, geometry, rows=None, chans=None):
+def _oracle_check(r, inputs, cube, geometry, rows=None, chans=None):
+    """r: the served request; inputs: its (lon, lat, values), which the
+    server released when the request completed."""
     import torch
+    lon_d, lat_d, values = inputs
     side = r.target.side
     m = side * side
     xw = np.empty(m)
@@ -127,8 +130,8 @@ def _oracle_check(r, cube, geometry, rows=None, chans=None):
     torch.cuda.synchronize()
     chans = list(range(r.nchan)) if chans is None else chans
     r0, r1 = (0, side) if rows is None else rows
-    V = r.values[chans].float().cpu().numpy()
-    lon, lat = r.lon.cpu().numpy(), r.lat.cpu().numpy()
+    V = values[chans].float().cpu().numpy()
+    lon, lat = lon_d.cpu().numpy(), lat_d.cpu().numpy()
     h, g = oracle.prepare_grid_kernel(oracle.GAUSS1D, [r.sigma], 3 * r.sigma, r.sigma / 2, side)
     keys, perm = oracle.presort(h, lon, lat)
     assert np.array_equal(geometry.perm(), perm)
@@ -150,8 +153,9 @@ def test_served_request_against_oracle(gpu):
     srv.submit(r)
     srv.serve(1, timeout_s=120)
     assert r.error is None
+    assert r.lon is None and r.values is None  # released by the server on completion
     print(f"case: served request n={r.n} side={r.target.side} nchan={r.nchan}", flush=True)
-    err = _oracle_check(r, r.cube.cpu().numpy(), srv.geoms[r.geo_key].geometry)
+    err = _oracle_check(r, (lon, lat, vals), r.cube.cpu().numpy(), srv.geoms[r.geo_key].geometry)
     srv.close()
     print(f"  errors {err}", flush=True)
     assert err["nan_match"] and err["max_rel"] <= 1e-4 and err["rms_rel"] <= 1e-4
@@ -164,6 +168,8 @@ def test_served_configured_mix_burst(gpu):
     reqs = [s.make() for s in specs]
     print(f"case: 60-request burst of the configured mix, {sum(s.repeat for s in specs)} repeats, "
           f"{sum(r.updates for r in reqs):.3e} updates", flush=True)
+    side_max = max(r.target.side for r in reqs)
+    keep = {r.rid: (r.lon, r.lat, r.values) for r in reqs if r.target.side == side_max}
     srv = serve.GriddingServer(keep_results=True, slots=2, budget_bytes=200e9)
     for r in reqs:
         srv.submit(r)
@@ -171,10 +177,10 @@ def test_served_configured_mix_burst(gpu):
     print(f"  server stats {dict(srv.stats)}", flush=True)
     assert all(r.error is None for r in reqs) and len(srv.finished) == 60
     # a 2048^2 request against the oracle (4 middle rows, every 32nd channel)
-    big = max((r for r in reqs if r.target.side == 2048), key=lambda r: r.updates)
+    big = max((r for r in reqs if r.target.side == side_max), key=lambda r: r.updates)
     side = big.target.side
     print(f"case: burst request {big.rid} n={big.n} nchan={big.nchan} side {side} against the oracle", flush=True)
-    err = _oracle_check(big, big.cube.cpu().numpy(), srv.geoms[big.geo_key].geometry,
+    err = _oracle_check(big, keep.pop(big.rid), big.cube.cpu().numpy(), srv.geoms[big.geo_key].geometry,
                         rows=(side // 2 - 2, side // 2 + 2), chans=list(range(0, big.nchan, 32)))
     print(f"  errors {err}", flush=True)
     assert err["nan_match"] and err["cells"] > 0
@@ -184,7 +190,7 @@ def test_served_configured_mix_burst(gpu):
     pick = [max(specs, key=lambda s: s.n * s.nchan)]
     pick += [s for s in specs if s.repeat][:1] + [s for s in specs if s.nchan == 1][:1]
     srv.close()
-    del reqs
+    del reqs, keep
     for s in pick:
         print(f"case: burst request {s.rid} alone (n={s.n} nchan={s.nchan} side={s.side})", flush=True)
         c0, _ = _alone(s)

@@ -146,12 +146,13 @@ def test_every_request_served_and_budget_held():
     be = FakeBackend()
     srv = _server(be, budget_bytes=3000, slots=2)
     reqs = [_req(i, n=100 + 10 * i, nchan=3) for i in range(12)]
+    vals = {r.rid: r.values for r in reqs}
     _run(srv, reqs)
     assert len(srv.finished) == 12 and not srv.failed
     assert be.max_used <= 3000
     for r in reqs:
         assert r.error is None
-        np.testing.assert_allclose(r.cube[:, 0], r.values.sum(axis=1, dtype=np.float64))
+        np.testing.assert_allclose(r.cube[:, 0], vals[r.rid].sum(axis=1, dtype=np.float64))
     # every round's memory returned: only the cached geometries stay charged
     assert srv.used == sum(e.bytes for e in srv.geoms.values())
     assert all(e.refs == 0 for e in srv.geoms.values())
@@ -161,13 +162,14 @@ def test_repeat_coordinates_hit_the_cache():
     be = FakeBackend()
     srv = _server(be, budget_bytes=10 ** 6, slots=1)
     base = _req(0)
-    reqs = [base] + [_req(i, coords=(base.lon.copy(), base.lat.copy())) for i in range(1, 5)]
+    lon, lat = base.lon.copy(), base.lat.copy()  # the server releases base's inputs when it completes
+    reqs = [base] + [_req(i, coords=(lon.copy(), lat.copy())) for i in range(1, 5)]
     _run(srv, reqs)
     assert srv.stats["geometry_builds"] == 1 and srv.stats["geometry_hits"] == 4
     assert len(be.built) == 1
     # same coordinates, another kernel width: another geometry
     srv2 = _server(FakeBackend(), budget_bytes=10 ** 6, slots=1)
-    _run(srv2, [_req(10, coords=(base.lon, base.lat)), _req(11, coords=(base.lon, base.lat), sigma=0.03)])
+    _run(srv2, [_req(10, coords=(lon, lat)), _req(11, coords=(lon, lat), sigma=0.03)])
     assert srv2.stats["geometry_builds"] == 2
 
 
@@ -178,12 +180,13 @@ def test_hit_geometry_is_pinned_against_eviction_in_the_same_pass():
     be = FakeBackend(polls=0)
     srv = _server(be, budget_bytes=1000, slots=1, max_round=8)
     a0 = _req(0, n=300, nchan=1)
+    lon0, lat0 = a0.lon.copy(), a0.lat.copy()
     _run(srv, [a0])  # builds G (300 bytes), now cached and unreferenced
     assert srv.used == 300
     srv.submit(_req(5, n=200, nchan=1))
     srv.serve(2, timeout_s=10)  # G and H (200) cached: 500 used
     assert srv.used == 500
-    a = _req(1, n=300, nchan=1, coords=(a0.lon.copy(), a0.lat.copy()))  # hit on G: needs 300
+    a = _req(1, n=300, nchan=1, coords=(lon0, lat0))  # hit on G: needs 300
     b = _req(2, n=150, nchan=1)  # miss: needs 150 + 150
     srv.submit(a)
     srv.submit(b)
@@ -275,3 +278,36 @@ def test_continuous_batching_overlaps_rounds():
     _run(srv, reqs)
     assert len(srv.finished) == 8
     assert srv.stats["rounds"] >= 4 and srv.stats["round_jobs_max"] <= 2
+
+
+def test_inputs_released_on_completion():
+    # served, failed at submit and failed at set-up: the server keeps no
+    # reference to any request's inputs once it has completed
+    import gc
+    import weakref
+    be = FakeBackend(fail_prepare={1})
+    srv = _server(be, budget_bytes=10 ** 6, slots=1)
+    reqs = [_req(i, n=50 + i, nchan=2) for i in range(4)]
+    reqs[2].dtype = "f32"  # invalid: fails at submit
+    refs = [weakref.ref(r.values) for r in reqs]
+    shapes = [(r.n, r.nchan) for r in reqs]
+    _run(srv, reqs)
+    be.rid_of.clear()  # the stand-in's own bookkeeping
+    gc.collect()
+    assert [r.rid for r in srv.finished] == [0, 3] and len(srv.failed) == 2
+    for r, ref, shp in zip(reqs, refs, shapes):
+        assert r.lon is None and r.lat is None and r.values is None
+        assert ref() is None, f"request {r.rid}: its values are still referenced"
+        assert (r.n, r.nchan) == shp and r.updates == shp[0] * shp[1]
+
+
+def test_abort_ends_the_loop():
+    # a producer that fails calls abort(): the loop raises instead of
+    # waiting for requests that will never arrive
+    import threading
+    be = FakeBackend()
+    srv = _server(be, budget_bytes=10 ** 6, slots=1)
+    srv.submit(_req(0))
+    threading.Timer(0.2, srv.abort, args=(RuntimeError("producer failed"),)).start()
+    with pytest.raises(RuntimeError, match="aborted after 1 of 5"):
+        srv.serve(5, timeout_s=30)
