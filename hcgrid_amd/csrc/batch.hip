@@ -401,8 +401,7 @@ int batch_init(uint32_t njobs, const hcg_batch_job* jobs, const BatchPlan& p, vo
                     bt->values_sorted[j],
                     b.cube,
                     b.wsum,
-                    g->kern.kernel_type == HCG_GAUSS1D && gauss_small_radius(P) ? HCG_GAUSS1D + 16u
-                                                                                 : g->kern.kernel_type,
+                    mfma_weight_form(P, g->kern.kernel_type),
                     hcg_channels_padded_len(g->n),
                     b.nchan,
                     (uint32_t)b.dtype,

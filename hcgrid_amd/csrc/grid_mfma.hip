@@ -68,21 +68,22 @@ __device__ inline f32x4_t mfma16(bool bf, s16x8_t a, s16x8_t b, f32x4_t c) {
 }
 
 // One k-block of B for kMcNB channel blocks, 16 bytes per lane per block.
-// The sorted channel values are blocked by 32 samples: block g holds
-// [nchan][32] values, so p points at block g + the lane's sample offset and
-// chofs[nb] = 32 x the lane's channel; a wave's 64 channels of a block are
-// one contiguous 4 KB piece.
-__device__ inline void load_b_rows(s16x8_t* b, const uint16_t* p, const uint32_t* chofs) {
+// The sorted channel values are blocked by kVB = 8 samples ([nchan][8] per
+// block): lane l's operand is 8 samples of one channel, a row of the value
+// block of its k-group (l / 16) in the k-block, so `blk` points at that
+// value block and lofs[nb] = 16 x the lane's channel; the 16 lanes of a
+// k-group read one contiguous 256-byte piece per channel block.
+__device__ inline void load_b_rows(s16x8_t* b, const char* blk, const uint32_t* lofs) {
 #pragma unroll
-  for (int nb = 0; nb < kMcNB; ++nb) b[nb] = *reinterpret_cast<const s16x8_t*>(p + chofs[nb]);
+  for (int nb = 0; nb < kMcNB; ++nb) b[nb] = *reinterpret_cast<const s16x8_t*>(blk + lofs[nb]);
 }
 
 // FP64 weight of a selected pair (the single-channel kernel's evaluation).
 __device__ inline double pair_weight_any(const GridParams& P, const GridBuffers& B, uint32_t kt, double c2,
                                          const double* s_exp, uint32_t cell, uint32_t i) {
-  // kt == HCG_GAUSS1D + 16: GAUSS1D whose radius rules out the exponent's
-  // asin branch and the exp clamp (gauss_small_radius): the same values
-  if (kt == HCG_GAUSS1D + 16u) return exp_scaled_neg_nc(gauss_exponent_series(P, c2), s_exp);
+  // GAUSS1D whose radius rules out the exponent's asin branch and the exp
+  // clamp (gauss_small_radius): the same values
+  if (kt == kKtGaussSmall || kt == kKtGaussLean) return exp_scaled_neg_nc(gauss_exponent_series(P, c2), s_exp);
   if (kt == HCG_GAUSS1D) return exp_scaled_neg(gauss_exponent_scaled(P, c2), s_exp);
   if (kt == HCG_GAUSS2D)
     return weight_general<HCG_GAUSS2D>(P.params[0], P.params[1], P.params[2], B.xwcs, B.ywcs, B.lon_s, B.lat_s,
@@ -195,29 +196,34 @@ int launch_mfma_plan(const MfmaJob& J, hipStream_t s) {
 // LDS of one workgroup (the weight tiles as raw 16-bit storage).
 template <uint32_t CHUNK>
 struct MfmaLds {
-  __attribute__((aligned(16))) uint16_t wh[kMcCols * (CHUNK + 8)];
-  __attribute__((aligned(16))) uint16_t wl[kMcCols * (CHUNK + 8)];
+  // first, so one address and immediate offsets reach a sample's x, y, z
   double x[CHUNK], y[CHUNK], z[CHUNK];
   uint32_t key[CHUNK];
+  // the weight tiles, row-interleaved: cell c's hi weights at c * RS, its lo
+  // weights at c * RS + CHUNK (elements; RS = 2 CHUNK + 8), so one address
+  // and an immediate offset reach both; RS is an odd multiple of 16 bytes,
+  // so the 16 rows of an MFMA operand read start in 16 distinct groups of 4
+  // banks
+  __attribute__((aligned(16))) uint16_t w[kMcCols * (2 * CHUNK + 8)];
   uint32_t mb[2][4][2];  // row blocks' k-block spans, by chunk parity
-  double wsum[kMcCols];
+  double inv[kMcCols];  // per cell: 1 / its weight sum (1 without normalisation)
   double p[kMcCols][3];          // cell unit vectors
   uint32_t span[2];              // the block's ring span
   uint32_t plan[kPlanStride];    // the tile's plan (when the job has one)
   // chunk descriptors (double-buffered): blocks and segments; per block its
   // sorted 32-sample block index, per segment its ring, first block, blocks
-  uint32_t dn[2][2];
-  uint32_t dgb[2][CHUNK / 32];
-  uint32_t dring[2][CHUNK / 32];
-  uint16_t du0[2][CHUNK / 32], dnb[2][CHUNK / 32];
+  uint32_t dn[2][3];                   // k-blocks, segments, value blocks
+  uint32_t dgb[2][CHUNK / kVB];        // per value block of the chunk: its sorted block index
+  uint32_t dring[2][CHUNK / 32];       // per segment (at most CHUNK / 32 per chunk): its ring,
+  uint16_t du0[2][CHUNK / 32], dnb[2][CHUNK / 32];  // first value block, value blocks
   uint32_t band[CHUNK / 32][kMcCols];  // each cell's band per segment: ja | jb << 16
   Window win[kMcCols];                 // each cell's window
 };
 
-// CHUNK = staged samples per chunk.  LDS: two weight tiles of 64 x (CHUNK+8)
-// elements (16-byte row padding) + CHUNK unit vectors and keys + the chunk
-// descriptors and bands: 157 KB at 512 (one workgroup per CU), 80 KB at 256
-// (two per CU), 47 KB at 128.
+// CHUNK = staged samples per chunk.  LDS: the weight tiles (64 rows of
+// 2 CHUNK + 8 elements) + CHUNK unit vectors and keys + the chunk
+// descriptors and bands: 156 KB at 512 (one workgroup per CU), 79 KB at 256
+// (two per CU), 46 KB at 128.
 //
 // The 64 cells of a workgroup form a TR x TC block of the map (TC = 1 <<
 // tc_shift columns, TR = 64 / TC rows); cell c of the block is (c / TC,
@@ -229,21 +235,23 @@ template <int DT, uint32_t CHUNK>
 __device__ __forceinline__ void mfma_block(const MfmaJob& J, uint32_t tile, uint32_t group, MfmaLds<CHUNK>& L,
                                            const double* s_exp) {
   typedef s16x8_t vec;
-  constexpr uint32_t WS = CHUNK + 8;  // weight-tile row stride (elements)
-  uint16_t* s_wh = L.wh;
-  uint16_t* s_wl = L.wl;
+  constexpr uint32_t WS = CHUNK;       // hi -> lo offset within a weight row (elements)
+  constexpr uint32_t RS = 2 * WS + 8;  // weight row stride (elements)
+  static_assert((WS * 2) % 16 == 0 && (RS * 2) % 16 == 0 && (RS / 2) % 8 == 4, "weight tile layout");
+  uint16_t* s_wh = L.w;
+  uint16_t* s_wl = L.w + WS;
   const bool bf = dt_is_bf16<DT>(J.dtype);
   double* s_x = L.x;
   double* s_y = L.y;
   double* s_z = L.z;
   uint32_t* s_key = L.key;
   auto& s_mb = L.mb;
-  double* s_wsum = L.wsum;
+  double* s_inv = L.inv;
   const GridParams& P = J.P;
   const GridBuffers& B = J.B;
   const uint16_t* __restrict__ vs = static_cast<const uint16_t*>(J.vs);
   const uint32_t kt = J.kt, nchan = J.nchan, tc_shift = J.tc_shift, tiles_x = J.tiles_x;
-  const size_t blk = (size_t)nchan * 32;  // elements per 32-sample block of the channel values
+  const size_t blk = (size_t)nchan * kVB;  // elements per value block
   float* __restrict__ cube = J.cube;
   float* __restrict__ wsum_out = J.wsum;
   const int normalize = J.normalize;
@@ -302,37 +310,56 @@ __device__ __forceinline__ void mfma_block(const MfmaJob& J, uint32_t tile, uint
   const uint32_t ch_base = (group * nw + wave) * kMcChanPerWave;
   // B operand addressing: lane l reads samples 8 (l / 16) .. +7 of a
   // k-block for channel ch_base + 16 nb + l % 16 (clamped to the last one)
-  const uint32_t koff = (lane >> 4) * 8;
-  uint32_t bofs[kMcNB];
+  const uint32_t kg = lane >> 4, koff = kg * 8;  // the lane's k-group: samples koff .. koff + 7 of a k-block
+  uint32_t lofs[kMcNB];
 #pragma unroll
   for (int nb = 0; nb < kMcNB; ++nb) {
     const uint32_t ch = ch_base + nb * 16 + (lane & 15);
-    bofs[nb] = (ch < nchan ? ch : nchan - 1) * 32u;
+    lofs[nb] = (ch < nchan ? ch : nchan - 1) * (kVB * 2u);
   }
+  // the lane's value block of k-block u of descriptor db
+  const char* vs_bytes = reinterpret_cast<const char*>(vs);
+  auto kblock = [&](uint32_t db, uint32_t u) {
+    return vs_bytes + (size_t)L.dgb[db][u * (32 / kVB) + kg] * (blk * 2);
+  };
 
   const uint32_t r_begin = L.span[0], r_end = L.span[1];
   const uint32_t n_all = m.n;       // sorted samples (records and keys beyond are not read)
-  constexpr uint32_t KB = CHUNK / 32;  // 32-sample blocks per chunk
+  constexpr uint32_t KV = CHUNK / kVB;    // value blocks per chunk
+  constexpr uint32_t KSEG = CHUNK / 32;   // segments per chunk
 
   auto planned = [&](uint32_t rr) { return J.plan != nullptr && (rr - r_begin) < kPlanRings; };
   auto plan_lo = [&](uint32_t rr) { return L.plan[2 + 2 * (rr - r_begin)]; };
   auto plan_hi = [&](uint32_t rr) { return L.plan[3 + 2 * (rr - r_begin)]; };
   // sorted index of chunk position p of descriptor db
-  auto gidx = [&](uint32_t db, uint32_t p) { return L.dgb[db][p >> 5] * 32u + (p & 31u); };
+  auto gidx = [&](uint32_t db, uint32_t p) { return L.dgb[db][p / kVB] * kVB + (p % kVB); };
 
   // ---- chunk descriptors -----------------------------------------------------
-  // A chunk packs the 32-sample blocks of consecutive rings' union ranges,
-  // at most KB blocks: ring r's blocks [lo_r >> 5, (hi_r + 31) >> 5), in
-  // ascending ring and block order, so the staged keys are sorted and each
-  // cell's band in a ring is one interval of the chunk.  Wave 0 builds
-  // them, one ahead, walking the rings from its cursor (ring cr at block
-  // offset cbo of its range [clo, chi)); a ring past the plan has its
-  // range searched here (every lane of wave 0 takes part).
+  // A chunk packs the value blocks of consecutive rings' union ranges, at
+  // most KV blocks in at most KSEG segments: ring r's blocks [lo_r / kVB,
+  // (hi_r + kVB - 1) / kVB), in ascending ring and block order, so the
+  // staged keys are sorted and each cell's band in a ring is one interval of
+  // the chunk.  The chunk's k-blocks (32 positions) are nkb = ceil(blocks /
+  // 4); the slots past the last block repeat it (their positions are staged
+  // empty, weight 0).  Wave 0 builds them, one ahead, walking the rings from
+  // its cursor (ring cr at block offset cbo of its range [clo, chi)); a ring
+  // past the plan has its range searched here (every lane of wave 0 takes
+  // part).
   uint32_t cr = r_begin, cbo = 0, clo = 0, chi = 0;
   bool crange = false;
+  constexpr uint32_t VPK = 32 / kVB;  // value blocks per k-block
+  static_assert(KSEG >= VPK && KV % VPK == 0, "chunk geometry");
   auto build_desc = [&](uint32_t db) {  // wave 0, uniform
-    uint32_t nkb = 0, nseg = 0;
-    while (nkb < KB && r_begin <= r_end && cr <= r_end) {
+    uint32_t nvb = 0, nseg = 0;
+    while (nvb < KV && r_begin <= r_end && cr <= r_end) {
+      // with at most VPK - 1 segments left, only complete the current
+      // k-block, then end the chunk: every chunk but a tile's last ends on a
+      // k-block boundary, so k-block j of a tile is always value blocks
+      // VPK j .. VPK j + VPK - 1 of the tile's block sequence whatever the
+      // chunk size, and the FP32 sums do not depend on it
+      const uint32_t need = (VPK - nvb % VPK) % VPK;
+      const bool closing = nseg + (VPK - 1) >= KSEG;
+      if (closing && need == 0) break;  // uniform
       if (!crange) {
         clo = chi = 0;
         if (planned(cr)) {
@@ -355,15 +382,16 @@ __device__ __forceinline__ void mfma_block(const MfmaJob& J, uint32_t tile, uint
         crange = false;
         continue;
       }
-      const uint32_t b0 = (clo >> 5) + cbo, b1 = (chi + 31) >> 5;
-      const uint32_t take = min(KB - nkb, b1 - b0);
+      const uint32_t b0 = clo / kVB + cbo, b1 = (chi + kVB - 1) / kVB;
+      uint32_t take = min(KV - nvb, b1 - b0);
+      if (closing) take = min(take, need);
       if (lane == 0) {
-        for (uint32_t t = 0; t < take; ++t) L.dgb[db][nkb + t] = b0 + t;
+        for (uint32_t t = 0; t < take; ++t) L.dgb[db][nvb + t] = b0 + t;
         L.dring[db][nseg] = cr;
-        L.du0[db][nseg] = (uint16_t)nkb;
+        L.du0[db][nseg] = (uint16_t)nvb;
         L.dnb[db][nseg] = (uint16_t)take;
       }
-      nkb += take;
+      nvb += take;
       ++nseg;
       if (b0 + take == b1) {
         ++cr;
@@ -372,19 +400,23 @@ __device__ __forceinline__ void mfma_block(const MfmaJob& J, uint32_t tile, uint
         cbo += take;
       }
     }
+    const uint32_t nkb = (nvb + VPK - 1) / VPK;
     if (lane == 0) {
+      for (uint32_t t = nvb; t < nkb * VPK; ++t) L.dgb[db][t] = L.dgb[db][nvb - 1];
       L.dn[db][0] = nkb;
       L.dn[db][1] = nseg;
+      L.dn[db][2] = nvb;
     }
   };
 
   // ---- per-chunk steps -------------------------------------------------
   // stage chunk positions [from, 32 nkb) of descriptor db into LDS (keys past
-  // the samples are the empty key, beyond every window)
-  auto stage = [&](uint32_t db, uint32_t from, uint32_t nkb) {
+  // the samples and past the chunk's nvb value blocks are the empty key,
+  // beyond every window)
+  auto stage = [&](uint32_t db, uint32_t from, uint32_t nkb, uint32_t nvb) {
     for (uint32_t p = from + tid; p < nkb * 32u; p += nthreads) {
       const uint32_t g = gidx(db, p);
-      if (g < n_all) {
+      if (g < n_all && p < nvb * kVB) {
         const hcg_sample q = B.rec[g];
         s_x[p] = q.x;
         s_y[p] = q.y;
@@ -401,8 +433,8 @@ __device__ __forceinline__ void mfma_block(const MfmaJob& J, uint32_t tile, uint
     const uint32_t width8 = nkb * 4;  // 16-byte groups per row
     for (uint32_t g = tid; g < kMcCols * width8; g += nthreads) {
       const uint32_t rr = g / width8, cc = (g % width8) * 8;
-      *reinterpret_cast<uint4*>(&s_wh[rr * WS + cc]) = uint4{0u, 0u, 0u, 0u};
-      *reinterpret_cast<uint4*>(&s_wl[rr * WS + cc]) = uint4{0u, 0u, 0u, 0u};
+      *reinterpret_cast<uint4*>(&s_wh[rr * RS + cc]) = uint4{0u, 0u, 0u, 0u};
+      *reinterpret_cast<uint4*>(&s_wl[rr * RS + cc]) = uint4{0u, 0u, 0u, 0u};
     }
   };
   // reset the row blocks' k-block spans of buffer db (read last by the
@@ -421,7 +453,7 @@ __device__ __forceinline__ void mfma_block(const MfmaJob& J, uint32_t tile, uint
       ring_cell_window(P, B, m, L.win[lane], L.dring[db][sg], a, b);
       uint32_t ja = 0, jb = 0;
       if (a <= b) {
-        const uint32_t p0 = (uint32_t)L.du0[db][sg] * 32u, p1 = p0 + (uint32_t)L.dnb[db][sg] * 32u;
+        const uint32_t p0 = (uint32_t)L.du0[db][sg] * kVB, p1 = p0 + (uint32_t)L.dnb[db][sg] * kVB;
         ja = lower_bound_u32(s_key, p0, p1, a);
         jb = upper_bound_u32(s_key, ja, p1, b);
         if (ja < jb) {
@@ -432,9 +464,48 @@ __device__ __forceinline__ void mfma_block(const MfmaJob& J, uint32_t tile, uint
       L.band[sg][lane] = ja | (jb << 16);
     }
   };
+  // the lean form (kKtGaussLean): the tile loop's p.q candidate test on
+  // h = 1 - p.q and gauss_weight_lean, the guard band after the weight
+  const double npx = -px, npy = -py, npz = -pz;
+  const double h_hi = 0.5 * P.c_hi, h_lo = 0.5 * P.c_lo, k0h = 2.0 * P.k0;
+  const bool lean = kt == kKtGaussLean && B.w_s == nullptr;
+  const double k1v = in_vgprs(4.0 * P.k1), e3v = in_vgprs(kExpE3);
+  // the weight wt of pair (c, j) as hi + lo in the value type (~2x its
+  // significand), its two halves stored in the weight tiles and summed
+  auto store_weight = [&](uint32_t j, double wt) {
+    const float wf = (float)wt;
+    const uint16_t hi = round16(wf, bf);
+    const float hif = widen16(hi, bf);
+    // wf - hif is exact (Sterbenz), and hif + lof is exact in FP32 (at most
+    // 2 x 11 significant bits, the second ~2^-8 below the first)
+    const uint16_t lo = round16(wf - hif, bf);
+    s_wh[c * RS + j] = hi;
+    s_wl[c * RS + j] = lo;
+    wsum += (double)(hif + widen16(lo, bf));
+  };
   // weights of this thread's cell over its bands in the chunk (after the
   // bands barrier)
   auto weigh = [&](uint32_t db, uint32_t nseg) {
+    if (lean) {  // uniform
+      for (uint32_t sg = 0; sg < nseg; ++sg) {
+        const uint32_t bd = L.band[sg][c];
+        const uint32_t ja = bd & 0xFFFFu, jb = bd >> 16;
+        for (uint32_t j = ja + sub; j < jb; j += S) {
+          const double h = fma(npx, s_x[j], fma(npy, s_y[j], fma(npz, s_z[j], 1.0)));
+          if (h < h_hi) {
+            double wt = gauss_weight_lean(h, k0h, k1v, e3v, s_exp);
+            if (__builtin_amdgcn_ballot_w64(!(h < h_lo))) {  // some lane in the guard band (rare)
+              asm volatile("; guard band" ::);                // (keeps the branch above a scalar one)
+              if (!(h < h_lo) && !radius_test_ref(B.xwcs, B.ywcs, B.lon_s, B.lat_s, cell, gidx(db, j),
+                                                  P.sphere_radius))
+                wt = 0.0;
+            }
+            store_weight(j, wt);
+          }
+        }
+      }
+      return;
+    }
     for (uint32_t sg = 0; sg < nseg; ++sg) {
       const uint32_t bd = L.band[sg][c];
       const uint32_t ja = bd & 0xFFFFu, jb = bd >> 16;
@@ -447,45 +518,49 @@ __device__ __forceinline__ void mfma_block(const MfmaJob& J, uint32_t tile, uint
           const bool need_i = kt == HCG_GAUSS2D || kt == HCG_TAPERED_SINC;
           double wt = pair_weight_any(P, B, kt, c2, s_exp, cell, need_i ? gidx(db, j) : 0u);
           if (B.w_s) wt *= B.w_s[gidx(db, j)];
-          // the weight as hi + lo in the value type: ~2x its significand
-          const uint16_t hi = round16((float)wt, bf);
-          const float hif = widen16(hi, bf);
-          const uint16_t lo = round16((float)(wt - (double)hif), bf);
-          s_wh[c * WS + j] = hi;
-          s_wl[c * WS + j] = lo;
-          wsum += (double)hif + (double)widen16(lo, bf);
+          store_weight(j, wt);
         }
       }
     }
   };
   // the contraction over the chunk's k-blocks; bn holds block 0's channel
   // values on entry
-  auto contract = [&](uint32_t db, uint32_t nkb, vec* bn) {
+  auto contract = [&](uint32_t db, uint32_t nkb, vec* b0) {
+    // the row blocks' k-block spans (wave-uniform: scalar compares)
     uint32_t mblo[4], mbhi[4];
 #pragma unroll
     for (int mb = 0; mb < 4; ++mb) {
-      mblo[mb] = s_mb[db][mb][0];
-      mbhi[mb] = s_mb[db][mb][1];
+      mblo[mb] = __builtin_amdgcn_readfirstlane(s_mb[db][mb][0]);
+      mbhi[mb] = __builtin_amdgcn_readfirstlane(s_mb[db][mb][1]);
     }
-    // software pipeline: block u's MFMAs run while block u+1's channel values load
-    for (uint32_t u = 0; u < nkb; ++u) {
-      vec bc[kMcNB];
-#pragma unroll
-      for (int nb = 0; nb < kMcNB; ++nb) bc[nb] = bn[nb];
-      if (u + 1 < nkb) load_b_rows(bn, vs + (size_t)L.dgb[db][u + 1] * blk + koff, bofs);
-      // A: the row block's hi and lo weights, from LDS
+    // block u's MFMAs on B operand b
+    auto mfmas = [&](uint32_t u, const vec* b) {
 #pragma unroll
       for (int mb = 0; mb < 4; ++mb) {
         if (!(u >= mblo[mb] && u < mbhi[mb])) continue;
-        const uint32_t off = (mb * 16 + (lane & 15)) * WS + u * 32 + koff;
+        // A: the row block's hi and lo weights, from LDS
+        const uint32_t off = (mb * 16 + (lane & 15)) * RS + u * 32 + koff;
         const vec ah = *reinterpret_cast<const vec*>(&s_wh[off]);
         const vec al = *reinterpret_cast<const vec*>(&s_wl[off]);
 #pragma unroll
         for (int nb = 0; nb < kMcNB; ++nb) {
-          acc[mb][nb] = mfma16(bf, ah, bc[nb], acc[mb][nb]);
-          acc[mb][nb] = mfma16(bf, al, bc[nb], acc[mb][nb]);
+          acc[mb][nb] = mfma16(bf, ah, b[nb], acc[mb][nb]);
+          acc[mb][nb] = mfma16(bf, al, b[nb], acc[mb][nb]);
         }
       }
+    };
+    // software pipeline over two register sets (no copies): block u's
+    // MFMAs run while block u+1's channel values load.  The loads are
+    // unconditional (past the last block they repeat it, unused), so every
+    // MFMA waits only for its own operand's loads
+    vec b1[kMcNB];
+    const uint32_t last = nkb - 1;
+    for (uint32_t u = 0; u < nkb; u += 2) {
+      load_b_rows(b1, kblock(db, min(u + 1, last)), lofs);
+      mfmas(u, b0);
+      if (u + 1 >= nkb) break;
+      load_b_rows(b0, kblock(db, min(u + 2, last)), lofs);
+      mfmas(u + 1, b1);
     }
   };
 
@@ -501,30 +576,30 @@ __device__ __forceinline__ void mfma_block(const MfmaJob& J, uint32_t tile, uint
   reset_spans(0);
   __syncthreads();
   for (;;) {
-    const uint32_t nkb = L.dn[db][0], nseg = L.dn[db][1];
+    const uint32_t nkb = L.dn[db][0], nseg = L.dn[db][1], nvb = L.dn[db][2];
     if (nkb == 0) break;  // uniform
     clear(nkb);
     reset_spans(db ^ 1u);
     if (staged < nkb * 32u) {  // uniform
-      stage(db, staged, nkb);
+      stage(db, staged, nkb, nvb);
       __syncthreads();  // B1
     }
     bands(db, nseg);
     __syncthreads();  // B2
     // channel values of the chunk's first k-block, in flight while the weights are computed
     vec bn[kMcNB];
-    if (ch_base < nchan) load_b_rows(bn, vs + (size_t)L.dgb[db][0] * blk + koff, bofs);
+    if (ch_base < nchan) load_b_rows(bn, kblock(db, 0), lofs);
     if (wave == 0) build_desc(db ^ 1u);
     weigh(db, nseg);
     __syncthreads();  // B3
-    const uint32_t nn = L.dn[db ^ 1u][0] * 32u;
+    const uint32_t nn = L.dn[db ^ 1u][0] * 32u, nvb_next = L.dn[db ^ 1u][2];
     staged = min(nn, nthreads);
     const bool pre = tid < staged;
     hcg_sample pq{};
     uint32_t pkey = 0xFFFFFFFFu;
     if (pre) {
       const uint32_t g = gidx(db ^ 1u, tid);
-      if (g < n_all) {
+      if (g < n_all && tid < nvb_next * kVB) {
         pq = B.rec[g];
         pkey = B.keys[g];
       }
@@ -543,13 +618,17 @@ __device__ __forceinline__ void mfma_block(const MfmaJob& J, uint32_t tile, uint
   // per-cell weight sums over the waves (exact in FP64 in any order: every
   // weight is hi + lo of two 16-bit values); the weight tiles are free now
   __syncthreads();
-  double* wpart = reinterpret_cast<double*>(L.wh);
+  double* wpart = reinterpret_cast<double*>(L.w);
   wpart[wave * kMcCols + lane] = wsum;
   __syncthreads();
   if (wave == 0) {
     double t = 0.0;
     for (uint32_t w = 0; w < nw; ++w) t += wpart[w * kMcCols + lane];
-    s_wsum[lane] = t;
+    // the normalisation as one multiply per value: v * (1 / t) is within an
+    // FP64 ulp or two of v / t, so the FP32 result is the quotient's except
+    // where it lies that close to an FP32 rounding boundary (1 ulp then);
+    // t = 0 gives inf, so an empty cell is 0 * inf = NaN as 0 / 0 is
+    s_inv[lane] = normalize ? 1.0 / t : 1.0;
     if (group == 0 && wsum_out && live) wsum_out[cell] = (float)t;
   }
   __syncthreads();
@@ -557,29 +636,38 @@ __device__ __forceinline__ void mfma_block(const MfmaJob& J, uint32_t tile, uint
   // cells of one map row (tile widths are multiples of 4), stored as one
   // 16-byte piece when the map side is a multiple of 4 (then aligned)
   if (ch_base < nchan) {
-    const bool vec4 = (P.side & 3u) == 0;
+    auto value = [&](int mb, int nb, int j, uint32_t cl0) {
+      return (float)((double)acc[mb][nb][j] * s_inv[cl0 + j]);
+    };
+    if ((P.side & 3u) == 0) {  // uniform
 #pragma unroll
-    for (int mb = 0; mb < 4; ++mb) {
-      const uint32_t cl0 = mb * 16 + (lane >> 4) * 4;
-      const uint32_t rowj = y0 + (cl0 >> tc_shift), col0 = x0 + (cl0 & tc_mask);
-      if (col0 >= P.side || rowj >= P.side) continue;
+      for (int mb = 0; mb < 4; ++mb) {
+        const uint32_t cl0 = mb * 16 + (lane >> 4) * 4;
+        const uint32_t rowj = y0 + (cl0 >> tc_shift), col0 = x0 + (cl0 & tc_mask);
+        if (col0 >= P.side || rowj >= P.side) continue;
 #pragma unroll
-      for (int nb = 0; nb < kMcNB; ++nb) {
-        const uint32_t ch = ch_base + nb * 16 + (lane & 15);
-        if (ch >= nchan) continue;
-        float o[4];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const double v = (double)acc[mb][nb][j];
-          o[j] = normalize ? (float)(v / s_wsum[cl0 + j]) : (float)v;
+        for (int nb = 0; nb < kMcNB; ++nb) {
+          const uint32_t ch = ch_base + nb * 16 + (lane & 15);
+          if (ch >= nchan) continue;
+          float* dst = cube + (size_t)ch * M + (size_t)rowj * P.side + col0;
+          *reinterpret_cast<float4*>(dst) =
+              make_float4(value(mb, nb, 0, cl0), value(mb, nb, 1, cl0), value(mb, nb, 2, cl0), value(mb, nb, 3, cl0));
         }
-        float* dst = cube + (size_t)ch * M + (size_t)rowj * P.side + col0;
-        if (vec4) {
-          *reinterpret_cast<float4*>(dst) = make_float4(o[0], o[1], o[2], o[3]);
-        } else {
+      }
+    } else {
+#pragma unroll
+      for (int mb = 0; mb < 4; ++mb) {
+        const uint32_t cl0 = mb * 16 + (lane >> 4) * 4;
+        const uint32_t rowj = y0 + (cl0 >> tc_shift), col0 = x0 + (cl0 & tc_mask);
+        if (col0 >= P.side || rowj >= P.side) continue;
+#pragma unroll
+        for (int nb = 0; nb < kMcNB; ++nb) {
+          const uint32_t ch = ch_base + nb * 16 + (lane & 15);
+          if (ch >= nchan) continue;
+          float* dst = cube + (size_t)ch * M + (size_t)rowj * P.side + col0;
 #pragma unroll
           for (int j = 0; j < 4; ++j)
-            if (col0 + j < P.side) dst[j] = o[j];
+            if (col0 + j < P.side) dst[j] = value(mb, nb, j, cl0);
         }
       }
     }
@@ -609,6 +697,10 @@ __global__ void __launch_bounds__(64 * kMcMaxWaves) k_grid_mfma(MfmaJob J) {
 // own kTileLdsBytes slice of the workgroup's LDS (the host keeps tpi within
 // tile_items_per_block).  Several streams: tile `item`, its streams over
 // the waves (one_channel_tile; the same sums for any number of waves).
+// two 256-sample workgroups (with the exp table and the queue slot) share a
+// CU's 160 KB of LDS; one 512-sample workgroup fills it
+static_assert(2 * (sizeof(MfmaLds<256>) + 64 * sizeof(double) + 16) <= 160 * 1024, "two CHUNK-256 workgroups per CU");
+static_assert(sizeof(MfmaLds<512>) + 64 * sizeof(double) + 16 <= 160 * 1024, "one CHUNK-512 workgroup per CU");
 static_assert(sizeof(MfmaLds<128>) >= one_channel_lds_bytes(kOneChannelStreams, kOneChannelStreams),
               "the smallest workgroup LDS holds a one-channel tile's streams");
 
@@ -741,18 +833,19 @@ int launch_mfma_batch(uint32_t chunk, uint32_t waves, uint32_t blocks, const Mfm
 }
 
 // Channel gather into the blocked layout: sorted slot i of channel ch goes
-// to block i / 32, row ch, column i % 32 ([nblk][nchan][32], zero-padded to
-// nblk = hcg_channels_padded_len(n) / 32 blocks).  A workgroup takes
-// kGatherSlots sorted slots x kGatherChans channels through LDS: thread t
-// gathers slot t of each channel (consecutive lanes read the caller's
-// values at consecutive permuted positions, 2 bytes each), and then writes
-// 16-byte pieces of block rows, a block's channel rows being contiguous.
+// to block i / 8, row ch, column i % 8 ([nblk][nchan][8], kVB = 8 slots per
+// block, zero-padded to nblk = hcg_channels_padded_len(n) / 8 blocks).  A
+// workgroup takes kGatherSlots sorted slots x kGatherChans channels through
+// LDS: thread t gathers slot t of each channel (consecutive lanes read the
+// caller's values at consecutive permuted positions, 2 bytes each), and then
+// writes block rows (16 bytes each), consecutive threads the consecutive
+// channel rows of one block.
 constexpr uint32_t kGatherChans = 32;
 constexpr uint32_t kGatherSlots = 256;
 constexpr uint32_t kGatherRow = kGatherSlots + 8;  // LDS row stride (16-byte pad against bank conflicts)
 
 __device__ inline size_t blocked_index(uint32_t i, uint32_t ch, uint32_t nchan) {
-  return ((size_t)(i >> 5) * nchan + ch) * 32u + (i & 31u);
+  return ((size_t)(i / kVB) * nchan + ch) * kVB + (i % kVB);
 }
 
 template <typename T>
@@ -789,16 +882,18 @@ __device__ inline void gather_cm_block(uint32_t bx, uint32_t by, uint32_t n, uin
   }
   if (nonfinite && bad) atomicAdd(nonfinite, bad);
   __syncthreads();
-  // pieces: (32-slot block kk, channel c, quarter q) -> 8 values; the
-  // output is 16-byte aligned when the buffer is (a row is 64 bytes)
+  // pieces: (block b of the workgroup's slots, channel c) -> one block row
+  // of kVB values; the output is 16-byte aligned when the buffer is (a row
+  // is 16 bytes)
+  static_assert(kVB == 8 && kGatherSlots % kVB == 0, "a block row is one 16-byte piece");
   const bool vec = (reinterpret_cast<uintptr_t>(out) & 15u) == 0;
-  const uint32_t npieces = (kGatherSlots / 32) * nc * 4;
+  const uint32_t npieces = (kGatherSlots / kVB) * nc;
   for (uint32_t p = t; p < npieces; p += 256) {
-    const uint32_t kk = p / (nc * 4), c = (p / 4) % nc, q = p % 4;
-    const uint32_t blk = (i0 >> 5) + kk;
-    if (blk * 32 >= n_pad) continue;
-    const uint16_t* src = lds + c * kGatherRow + kk * 32 + q * 8;
-    T* dst = out + ((size_t)blk * nchan + ch0 + c) * 32 + q * 8;
+    const uint32_t b = p / nc, c = p % nc;
+    const uint32_t blk = i0 / kVB + b;
+    if (blk * kVB >= n_pad) continue;
+    const uint16_t* src = lds + c * kGatherRow + b * kVB;
+    T* dst = out + ((size_t)blk * nchan + ch0 + c) * kVB;
     if (vec) {
       *reinterpret_cast<uint4*>(dst) = *reinterpret_cast<const uint4*>(src);
     } else {
@@ -1050,7 +1145,7 @@ extern "C" int hcg_grid_mfma_ex(const hcg_healpix* hp, const hcg_kernel* kern, u
   // chunk size by workgroup width: 8 waves fill a CU alone (VGPR-limited);
   // narrower workgroups take smaller tiles so several share a CU
   const uint32_t chunk = sh.chunk ? sh.chunk : (nw >= 8 ? 512u : (nw >= 4 ? 256u : 128u));
-  const uint32_t ktc = kern->kernel_type == HCG_GAUSS1D && gauss_small_radius(P) ? HCG_GAUSS1D + 16u : kern->kernel_type;
+  const uint32_t ktc = mfma_weight_form(P, kern->kernel_type);
   MfmaJob J{P,        B,       values_sorted, cube,   wsum, ktc, n_pad, nchan, (uint32_t)dtype,
             tc_shift, tiles_x, ntiles,        (uint32_t)(normalize ? 1 : 0), nullptr, 0u, 0u};
   if (workspace) {

@@ -227,8 +227,10 @@ inline bool gauss_small_radius(const GridParams& P) {
   return y > -90000.0;
 }
 
-// The tile loop's candidate test on p.q (c2 = 2 - 2 p.q, four FP64 operations
-// instead of seven for |p - q|^2): its c2 carries ~1.2e-15 absolute error,
+// The tile loop's candidate test on p.q (c2 = 2 h, h = 1 - p.q by three FMAs
+// from 1, instead of seven operations for |p - q|^2): the FMAs' two
+// roundings of O(1) partial sums and the unit vectors' own rounding leave
+// c2 within ~1.2e-15 absolute,
 // which at c_hi >= 1e-7 (R >= ~0.018 deg) is <= 1.2e-8 of the threshold, so
 // a weight exp(-p0 d^2) (p0 d^2 <= 4.5 at R = 3 sigma) moves by <= ~5e-8
 // relative; smaller radii keep |p - q|^2.  Selection is exact either way
@@ -273,10 +275,12 @@ constexpr double kExpE3 = kExpE2 * kExpE1 / 3.0;
 // term is below 5e-12 of the exponent there), and exp by the 64-entry table
 // (Tang 1989) with the degree-3 polynomial of exp(r) - 1, |r| <= ln2/128
 // (truncation r^4/24 < 3.6e-11 relative).  The weight is within 4e-11
-// relative of exp(-p0 d^2).  k1v and e3v hold k1 and kExpE3 in vector
-// registers (in_vgprs): every step is one v_fma_f64.
-__device__ inline double gauss_weight_lean(double c2, double k0, double k1v, double e3v, const double* s_exp) {
-  const double y = c2 * fma(c2, k1v, k0);
+// relative of exp(-p0 d^2).  The caller passes h = c2 / 2 with a = 2 k0
+// and b = 4 k1 (y = h (a + b h) is the same value: every scaling is by a
+// power of two); bv and e3v hold b and kExpE3 in vector registers
+// (in_vgprs), so every step is one v_fma_f64.
+__device__ inline double gauss_weight_lean(double h, double a, double bv, double e3v, const double* s_exp) {
+  const double y = h * fma(h, bv, a);
   const double n = rint(y);
   const int ni = (int)n;
   const double u = y - n;
@@ -375,8 +379,8 @@ inline GridParams make_grid_params(const hcg_healpix* hp, const hcg_kernel* kern
   P.factor = kern->selection == HCG_SELECT_DISC ? 1u : kern->factor;
   // radius threshold on c2 = 4 sin^2(R/2), with a guard band where the
   // reference's haversine decides: 1e-9 relative (c2 = |p - q|^2 is good to
-  // ~1e-12 relative) plus 4e-15 absolute (c2 = 2 - 2 p.q, the form the tile
-  // loop uses at dot_distance_ok radii, is good to ~1.2e-15 absolute)
+  // ~1e-12 relative) plus 4e-15 absolute (c2 = 2 (1 - p.q), the form the
+  // tile loop uses at dot_distance_ok radii, is good to ~1.2e-15 absolute)
   const double half = kern->sphere_radius * kDeg2Rad / 2.0;
   const double cR = 4.0 * sin(half) * sin(half);
   P.c_lo = cR * (1.0 - 1e-9) - 4e-15;

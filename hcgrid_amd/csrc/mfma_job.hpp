@@ -23,6 +23,19 @@ constexpr uint32_t kMcChanPerWave = 16 * kMcNB;
 constexpr uint32_t kPlanRings = 64;
 constexpr uint32_t kPlanStride = 2 + 2 * kPlanRings;
 
+// The weight form of a job (MfmaJob::kt): the kernel type, or
+// kKtGaussSmall (GAUSS1D whose radius rules out the exponent's asin branch
+// and the exp clamp, gauss_small_radius), or kKtGaussLean (also
+// gauss_tiny_radius and dot_distance_ok: the p.q candidate test and
+// gauss_weight_lean of the tile loop).
+constexpr uint32_t kKtGaussSmall = HCG_GAUSS1D + 16u;
+constexpr uint32_t kKtGaussLean = HCG_GAUSS1D + 32u;
+
+inline uint32_t mfma_weight_form(const GridParams& P, int kernel_type) {
+  if (kernel_type != HCG_GAUSS1D || !gauss_small_radius(P)) return (uint32_t)kernel_type;
+  return gauss_tiny_radius(P) && dot_distance_ok(P) ? kKtGaussLean : kKtGaussSmall;
+}
+
 // One multi-channel job: geometry, sorted channel values and outputs, with
 // its launch shape (tiles of (64 >> tc_shift) x (1 << tc_shift) cells).
 struct MfmaJob {
@@ -31,7 +44,7 @@ struct MfmaJob {
   const void* vs;     // sorted channel values [nchan][n_pad], BF16 or FP16
   float* cube;        // [nchan][side * side]
   float* wsum;        // [side * side] or NULL
-  uint32_t kt, n_pad, nchan, dtype;  // kt: kernel type; HCG_GAUSS1D + 16 = GAUSS1D at a small radius
+  uint32_t kt, n_pad, nchan, dtype;  // kt: mfma_weight_form
   uint32_t tc_shift, tiles_x, ntiles, normalize;
   uint32_t* plan;     // [ntiles][kPlanStride], or NULL (search in the gridding kernel)
   // 0: the MFMA contraction; else a one-channel job on the FP64 tile path

@@ -58,6 +58,12 @@
 
 namespace hcg {
 
+// Sorted channel values (multi-channel jobs) are blocked by kVB samples:
+// block b holds the [nchan][kVB] values of sorted samples kVB b .. kVB b +
+// kVB - 1, so a lane's 16-byte MFMA operand (8 samples of one channel) is
+// one block row, and the MFMA path packs ring ranges at this granularity.
+constexpr uint32_t kVB = 8;
+
 constexpr int kTileCols = HCG_GRID_TILE_COLS;
 constexpr int kLaneRows = 64 / kTileCols;
 constexpr int kCellsPerLane = HCG_GRID_CELLS_PER_LANE;
@@ -174,6 +180,9 @@ __device__ __forceinline__ void grid_tile(const GridParams P, const GridBuffers 
   static_assert(WMODE != 3 || (kCellsPerLane == 1 && !HCG_GRID_LANE_LOOP && KT == HCG_GAUSS1D),
                 "the p.q candidate test is written for the one-cell-per-lane GAUSS1D loop");
   static_assert(NC == 0 || (BATCH && NC % 2 == 0), "few-channel tiles: an even channel count, batch outputs");
+  // the p.q loop with one value per sample: the lean weight and the guard
+  // band handled after it (below)
+  constexpr bool kLean = WMODE == 3 && NC == 0;
   float acc_c[NC > 0 ? NC : 1];
 #pragma unroll
   for (int ch = 0; ch < (NC > 0 ? NC : 1); ++ch) acc_c[ch] = 0.0f;
@@ -264,12 +273,12 @@ __device__ __forceinline__ void grid_tile(const GridParams P, const GridBuffers 
       }
       if constexpr (NC > 0) {
         // channel by channel: lanes read consecutive samples of one channel
-        // (contiguous within a 32-sample block); slots past nchan hold 0
+        // (contiguous within a kVB-sample block); slots past nchan hold 0
         for (uint32_t ch = 0; ch < (uint32_t)NC; ++ch) {
           const bool have = ch < O.nchan;
           for (uint32_t j = lane; j < cnt; j += 64) {
             const uint32_t i = cb + j;
-            s_c[j * NC + ch] = have ? O.vs[((size_t)(i >> 5) * O.nchan + ch) * 32 + (i & 31)] : (uint16_t)0;
+            s_c[j * NC + ch] = have ? O.vs[((size_t)(i / kVB) * O.nchan + ch) * kVB + (i % kVB)] : (uint16_t)0;
           }
         }
       }
@@ -306,14 +315,25 @@ __device__ __forceinline__ void grid_tile(const GridParams P, const GridBuffers 
           const double dx = px[k] - qx, dy = py[k] - qy, dz = pz[k] - qz;
           const double c2 = fma(dx, dx, fma(dy, dy, dz * dz));
           if (c2 < P.c_hi) {
+            const double gq = c2, glo = P.c_lo;
 #elif HCG_GRID_CELLS_PER_LANE == 1
       // one cell per lane: its band is exactly [lo, lo + len)
       (void)jb;
       (void)je;
-      const double dot_lo = fma(-0.5, P.c_hi, 1.0);  // c2 < c_hi  <=>  p.q > 1 - c_hi / 2
-      (void)dot_lo;
-      // WMODE 3: the lean weight's constants in vector registers (hoisted out of the loops)
-      const double k1v = WMODE == 3 ? in_vgprs(P.k1) : 0.0, e3v = WMODE == 3 ? in_vgprs(kExpE3) : 0.0;
+      // WMODE 3: the tests on h = 1 - p.q = c2 / 2 (halving is exact, so
+      // h < c_hi / 2 <=> c2 < c_hi); -p in registers, so h is three FMAs
+      const double h_hi = 0.5 * P.c_hi, h_lo = 0.5 * P.c_lo;
+      const double npx = -px[0], npy = -py[0], npz = -pz[0];
+      (void)h_hi;
+      (void)npx;
+      (void)npy;
+      (void)npz;
+      // the lean weight's constants for y = h (2 k0 + 4 k1 h), which equals
+      // c2 (k0 + k1 c2) bit for bit: k1 and kExpE3 in vector registers
+      // (hoisted out of the loops)
+      const double k0h = 2.0 * P.k0;
+      const double k1v = WMODE == 3 ? in_vgprs(4.0 * P.k1) : 0.0, e3v = WMODE == 3 ? in_vgprs(kExpE3) : 0.0;
+      (void)k0h;
       (void)k1v;
       (void)e3v;
       for (uint32_t t = 0; t < len[0]; ++t) {
@@ -321,17 +341,18 @@ __device__ __forceinline__ void grid_tile(const GridParams P, const GridBuffers 
         const double qx = s_x[j], qy = s_y[j], qz = s_z[j], qv = NC > 0 ? 0.0 : s_v[j];
         {
           constexpr int k = 0;
-          // WMODE 3: the candidate test on p.q and c2 = 2 - 2 p.q (four FP64
+          // WMODE 3: the candidate test on h = 1 - p.q (three FP64
           // operations instead of seven); otherwise c2 = |p - q|^2
-          double c2, dot = 0.0;
+          double c2, h = 0.0;
           if constexpr (WMODE == 3) {
-            dot = fma(px[k], qx, fma(py[k], qy, pz[k] * qz));
+            h = fma(npx, qx, fma(npy, qy, fma(npz, qz, 1.0)));
           } else {
             const double dx = px[k] - qx, dy = py[k] - qy, dz = pz[k] - qz;
             c2 = fma(dx, dx, fma(dy, dy, dz * dz));
           }
-          if (WMODE == 3 ? dot > dot_lo : c2 < P.c_hi) {
-            if constexpr (WMODE == 3) c2 = fma(-2.0, dot, 2.0);
+          if (WMODE == 3 ? h < h_hi : c2 < P.c_hi) {
+            if constexpr (WMODE == 3) c2 = 2.0 * h;
+            const double gq = WMODE == 3 ? h : c2, glo = WMODE == 3 ? h_lo : P.c_lo;
 #else
       for (uint32_t j = jb; j < je; ++j) {
         const double qx = s_x[j], qy = s_y[j], qz = s_z[j], qv = s_v[j];
@@ -340,12 +361,30 @@ __device__ __forceinline__ void grid_tile(const GridParams P, const GridBuffers 
           const double dx = px[k] - qx, dy = py[k] - qy, dz = pz[k] - qz;
           const double c2 = fma(dx, dx, fma(dy, dy, dz * dz));
           if ((j - lo[k]) < len[k] && c2 < P.c_hi) {
+            const double gq = c2, glo = P.c_lo;
 #endif
-            if (c2 < P.c_lo || radius_test_ref(B.xwcs, B.ywcs, B.lon_s, B.lat_s, cell[k], cb + j, P.sphere_radius)) {
+            if constexpr (kLean) {
+              // every candidate's weight; a lane in the guard band (rare, and
+              // wave-uniformly skipped) that the reference's test rejects then
+              // adds zero: one compare and no lane mask in the common case
+              double wt = gauss_weight_lean(h, k0h, k1v, e3v, s_exp);
+              double qw = qv;
+              if constexpr (HAS_W) wt = B.w_s[cb + j] * wt;
+              if (__builtin_amdgcn_ballot_w64(!(gq < glo))) {
+                // (a block of its own, so the branch above stays a scalar one)
+                asm volatile("; guard band" ::);
+                if (!(gq < glo) &&
+                    !radius_test_ref(B.xwcs, B.ywcs, B.lon_s, B.lat_s, cell[k], cb + j, P.sphere_radius)) {
+                  wt = 0.0;
+                  qw = 0.0;
+                }
+              }
+              acc_d[k] = fma(qw, wt, acc_d[k]);
+              acc_w[k] += wt;
+            } else if (gq < glo ||
+                       radius_test_ref(B.xwcs, B.ywcs, B.lon_s, B.lat_s, cell[k], cb + j, P.sphere_radius)) {
               double wt;
-              if constexpr (KT == HCG_GAUSS1D && WMODE == 3 && HCG_GRID_CELLS_PER_LANE == 1 && !HCG_GRID_LANE_LOOP)
-                wt = gauss_weight_lean(c2, P.k0, k1v, e3v, s_exp);
-              else if constexpr (KT == HCG_GAUSS1D && WMODE > 0)
+              if constexpr (KT == HCG_GAUSS1D && WMODE > 0)
                 wt = gauss_weight_series<WMODE == 3 ? 2 : WMODE>(P, c2, s_exp);
               else if constexpr (KT == HCG_GAUSS1D)
                 wt = HCG_GRID_TABLE_EXP ? exp_scaled_neg(gauss_exponent_scaled(P, c2), s_exp)

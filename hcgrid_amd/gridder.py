@@ -273,7 +273,7 @@ class CubeGridder:
 
     Keys, sort, LUT and records as the single-channel path; channel values
     (bf16/f16, channel-major [nchan][n] or sample-major [n][nchan]) are
-    gathered into sorted order (blocked by 32 samples) and contracted on MFMA.
+    gathered into sorted order (blocked by 8 samples) and contracted on MFMA.
     """
 
     def __init__(self, n: int, side: int, nchan: int, kern: Kernel, hp: Healpix, dtype: str = "bf16",

@@ -200,9 +200,10 @@ int hcg_grid_plain(const hcg_healpix *hp, const hcg_kernel *kern, uint32_t side,
 
 /* Padded samples per channel of the sorted channel buffer: n rounded up to
  * 32 plus 64 zero samples.  The buffer holds nchan * hcg_channels_padded_len(n)
- * values, blocked by 32 sorted samples: [padded_len / 32][nchan][32], so the
- * contraction reads each 32-sample block of 64 channels as one contiguous
- * 4 KB piece. */
+ * values, blocked by 8 sorted samples: [padded_len / 8][nchan][8], so one
+ * lane's MFMA operand (8 samples of one channel) is one 16-byte block row, 16
+ * channels of a block are one contiguous 256-byte piece, and the contraction
+ * packs ring ranges at 8-sample granularity. */
 uint32_t hcg_channels_padded_len(uint32_t n);
 
 /* The per-channel permutation gather of HCGrid+'s pre_order_data

@@ -136,3 +136,45 @@ def test_plus_cli_hdf5_cube(gpu, tmp_path, cube_dtype):
     assert skv["NAXIS3"] == "5"
     assert np.array_equal(sv.reshape(5, n), V[:, perm])
     assert np.array_equal(sc[:n], lon[perm]) and np.array_equal(sc[n:], lat[perm])
+
+
+def test_plus_cli_reference_target_1895(gpu, tmp_path):
+    # the reference's HCGrid+ target fixture (test_data/HDF5/hdf_target1895.fits:
+    # a 500 x 500 x 1 SIN cube header at 0.01 deg around (60, 30)); its HDF5
+    # input is not in the reference, so a raster over the field (about 200k
+    # samples, 3 channels) is written with the library's own HDF5 writer.
+    # Every channel of the output cube against the oracle's FP64 pipeline.
+    shutil.copy(os.path.join(ROOT, "tests", "golden", "hdf_target1895.fits"), tmp_path / "hdf_target1895.fits")
+    tk, _ = read_fits(str(tmp_path / "hdf_target1895.fits"), 0)
+    assert tk["NAXIS"] == "3" and tk["NAXIS1"] == "500" and tk["NAXIS2"] == "500"
+    nx, ny = int(tk["NAXIS1"]), int(tk["NAXIS2"])
+    ra0, dec0 = float(tk["CRVAL1"]), float(tk["CRVAL2"])
+    rng = np.random.default_rng(1895)
+    dec = np.arange(dec0 - 2.7, dec0 + 2.7, 0.015)
+    lon = np.concatenate([np.arange(ra0 - 2.7, ra0 + 2.7, 0.01) for _ in dec])
+    per = lon.size // dec.size
+    lat = np.repeat(dec, per)
+    lon = ra0 + (lon - ra0) / np.cos(np.radians(lat)) + rng.uniform(-0.002, 0.002, lon.size)
+    n = lon.size
+    base = np.exp(-((lon - ra0) ** 2 + (lat - dec0) ** 2) / 2.0) * 10.0
+    V = np.stack([base + rng.standard_normal(n) * s for s in (0.1, 0.5, 1.0)])
+    _write_h5(str(tmp_path / "input1895.hdf5"), lon, lat, V)
+    d = str(tmp_path) + "/"
+    print(f"case: HCGridPlus on hdf_target1895.fits, n={n}, 3 channels", flush=True)
+    p = subprocess.run([EXE_PLUS, "--fits_path", d, "--input_file", "input", "--target_file", "hdf_target",
+                        "--output_file", "output", "--sorted_file", "sorted", "--fits_id", "1895",
+                        "--beam_size", "300", "--order_arg", "1", "--cube_dtype", "f64"],
+                       capture_output=True, text=True, timeout=300)
+    print(p.stdout, p.stderr, flush=True)
+    assert p.returncode == 0 and "channels=3" in p.stdout
+    kv, m = read_fits(str(tmp_path / "output1895.fits"), 0)
+    assert kv["NAXIS1"] == "500" and kv["NAXIS2"] == "500" and kv["NAXIS3"] == "3"
+    xw, yw = O.wcs_sin(nx, ny, float(tk["CRPIX1"]), float(tk["CRPIX2"]), float(tk["CDELT1"]),
+                       float(tk["CDELT2"]), ra0, dec0)
+    sigma = (300.0 / 3600.0) / math.sqrt(8 * math.log(2))
+    cube = m.reshape(3, nx * ny)
+    for c in range(3):
+        r = O.pipeline(lon, lat, V[c], xw, yw, ny, sigma, 5 * sigma, sigma / 2)
+        e = map_errors(cube[c], normalise(r["datacube"], r["weightscube"]))
+        print(f"  channel {c}: {e}", flush=True)
+        assert e["nan_match"] and e["max_rel"] <= 1e-6 and e["cells"] > 200000

@@ -92,6 +92,33 @@ def test_cube_launch_shapes(gpu, shape):
     _run_case(30, 160, 27, 100, "bf16", 0, seed=23, shape=shape)
 
 
+def test_cube_chunk_and_waves_change_no_bit(gpu):
+    # a dense raster (rings of hundreds of samples, several chunks per block
+    # at every chunk size): with the block shape fixed, the chunk size and
+    # the wave count change nothing, since chunks end on k-block boundaries
+    import torch
+    wl = synth.raster_small(40, 2400, 24, spacing_ra=0.0005, seed=17)
+    t = wl.target
+    xw, yw = gridder.wcs_sin(24, 24, t.crpix, t.crpix, -t.cdelt, t.cdelt, t.crval1, t.crval2)
+    Vt = torch.tensor(_values(wl, 96, 17)).to(torch.bfloat16).cuda()
+    kern, hp = gridder.prepare_grid_kernel(1, [wl.sigma], wl.radius, wl.hpx_res)
+    cg = gridder.CubeGridder(wl.n, 24, 96, kern, hp, dtype="bf16")
+    d = lambda a: torch.as_tensor(a, device="cuda")  # noqa: E731
+    xwd, ywd = d(xw), d(yw)
+    cg.run(d(wl.lon), d(wl.lat), Vt, xwd, ywd)
+    got = {}
+    for shp in [(16, 1, 128), (16, 2, 256), (16, 4, 256), (16, 8, 512), (16, 4, 128)]:
+        print(f"case: n={wl.n} side=24 nchan=96 shape={shp}", flush=True)
+        cg.grid(xwd, ywd, shape=shp)
+        torch.cuda.synchronize()
+        got[shp] = (cg.cube.cpu().numpy().copy(), cg.wsum.cpu().numpy().copy())
+    c0, w0 = got[(16, 8, 512)]
+    assert np.isfinite(c0).sum() > 1000
+    for shp, (c, w) in got.items():
+        assert np.array_equal(c, c0, equal_nan=True), f"shape {shp} changes the cube"
+        assert np.array_equal(w, w0), f"shape {shp} changes the weight sums"
+
+
 def test_cube_f16(gpu):
     _run_case(40, 200, 24, 96, "f16", 0, seed=3)
 
@@ -223,7 +250,7 @@ def test_cube_tile_plan_equals_search(gpu, sigma, radius, shape):
                                                   (64, 1, "f16", 0), (4100, 40, "f16", 1)])
 def test_gather_channels_exact(gpu, n, nchan, dtype, layout):
     # the blocked layout, element for element: slot i of channel ch at
-    # ((i // 32) * nchan + ch) * 32 + i % 32, zeros past n, and a count of the
+    # ((i // 8) * nchan + ch) * 8 + i % 8, zeros past n, and a count of the
     # non-finite values among the n samples (planted NaNs and infinities)
     import ctypes as C
 
@@ -253,7 +280,7 @@ def test_gather_channels_exact(gpu, n, nchan, dtype, layout):
     want = np.zeros(n_pad * nchan, dtype=np.int16)
     i = np.arange(n)
     for ch in range(nchan):
-        want[((i // 32) * nchan + ch) * 32 + i % 32] = bits[ch, perm]
+        want[((i // 8) * nchan + ch) * 8 + i % 8] = bits[ch, perm]
     assert np.array_equal(got, want)
     assert int(cnt.item()) == len(bad)
 
