@@ -357,7 +357,7 @@ int plan_batch(uint32_t njobs, const hcg_batch_job* jobs, uint32_t waves, uint32
     p.voff[j] = c.off;
     c.off += hcg_batch_job_bytes(jobs[j].geometry, jobs[j].nchan);
     p.poff[j] = c.off;
-    if (!p.tpi[j]) c.off = align_up(c.off + (size_t)p.tiles[j] * kPlanStride * sizeof(uint32_t));
+    if (!p.tpi[j]) c.off = align_up(c.off + (size_t)p.tiles[j] * kPlanTileWords * sizeof(uint32_t));
   }
   p.bytes = c.off;
   return HCG_OK;

@@ -160,11 +160,16 @@ __device__ inline void plan_tile(const MfmaJob& J, uint32_t tile) {
   double q[3];
   uint32_t r_begin, r_end;
   const Window win = block_cell_window(J.P, J.B, m, y0, x0, tc_shift, lane, q, r_begin, r_end);
-  uint32_t* out = J.plan + (size_t)tile * kPlanStride;
+  uint32_t* out = J.plan + (size_t)tile * kPlanTileWords;
   if (lane == 0) {
     out[0] = r_begin;
     out[1] = r_end;
   }
+  reinterpret_cast<Window*>(out + kPlanHeadWords)[lane] = win;
+  double* pq = reinterpret_cast<double*>(out + kPlanHeadWords + kPlanWinWords);
+  pq[lane] = q[0];
+  pq[kMcCols + lane] = q[1];
+  pq[2 * kMcCols + lane] = q[2];
   for (uint32_t k = 0; k < kPlanRings; ++k) {
     const uint32_t r = r_begin + k;
     uint32_t lo = 0, hi = 0;
@@ -283,7 +288,21 @@ __device__ __forceinline__ void mfma_block(const MfmaJob& J, uint32_t tile, uint
   // vector, the block's ring span, and per ring the cells' [lpix, rpix] and
   // the union's sorted range, published through LDS.  The other waves only
   // stage, weigh and contract.
-  if (wave == 0) {
+  if (J.plan) {  // the plan holds all of it (one round of coalesced loads)
+    const uint32_t* tp = J.plan + (size_t)tile * kPlanTileWords;
+    for (uint32_t i = tid; i < kPlanStride; i += nthreads) L.plan[i] = tp[i];
+    if (wave == 0) {
+      L.win[lane] = reinterpret_cast<const Window*>(tp + kPlanHeadWords)[lane];
+      const double* pq = reinterpret_cast<const double*>(tp + kPlanHeadWords + kPlanWinWords);
+      L.p[lane][0] = pq[lane];
+      L.p[lane][1] = pq[kMcCols + lane];
+      L.p[lane][2] = pq[2 * kMcCols + lane];
+      if (lane == 0) {
+        L.span[0] = tp[0];
+        L.span[1] = tp[1];
+      }
+    }
+  } else if (wave == 0) {
     double q[3];
     uint32_t ur, dr;
     L.win[lane] = block_cell_window(P, B, m, y0, x0, tc_shift, lane, q, ur, dr);
@@ -295,8 +314,6 @@ __device__ __forceinline__ void mfma_block(const MfmaJob& J, uint32_t tile, uint
       L.span[1] = dr;
     }
   }
-  if (J.plan)
-    for (uint32_t i = tid; i < kPlanStride; i += nthreads) L.plan[i] = J.plan[(size_t)tile * kPlanStride + i];
   __syncthreads();
   const double px = L.p[c][0], py = L.p[c][1], pz = L.p[c][2];
   double wsum = 0.0;
@@ -1074,7 +1091,7 @@ extern "C" size_t hcg_grid_mfma_workspace_size(uint32_t side, const hcg_mfma_sha
   const uint32_t tc = (shape && shape->tile_cols) ? shape->tile_cols : kMcDefaultTileCols;
   if (!(tc == 8 || tc == 16 || tc == 32 || tc == 64)) return 0;
   const size_t ntiles = (size_t)ceil_div_u32(side, tc) * ceil_div_u32(side, 64u / tc);
-  return ntiles * kPlanStride * sizeof(uint32_t);
+  return ntiles * kPlanTileWords * sizeof(uint32_t);
 }
 
 extern "C" int hcg_grid_mfma_ex(const hcg_healpix* hp, const hcg_kernel* kern, uint32_t side, uint32_t n,
@@ -1149,7 +1166,7 @@ extern "C" int hcg_grid_mfma_ex(const hcg_healpix* hp, const hcg_kernel* kern, u
   MfmaJob J{P,        B,       values_sorted, cube,   wsum, ktc, n_pad, nchan, (uint32_t)dtype,
             tc_shift, tiles_x, ntiles,        (uint32_t)(normalize ? 1 : 0), nullptr, 0u, 0u};
   if (workspace) {
-    if (workspace_bytes < (size_t)ntiles * kPlanStride * sizeof(uint32_t))
+    if (workspace_bytes < (size_t)ntiles * kPlanTileWords * sizeof(uint32_t))
       return set_error(HCG_ERR_WORKSPACE, "hcg_grid_mfma_ex: workspace smaller than hcg_grid_mfma_workspace_size");
     J.plan = static_cast<uint32_t*>(workspace);
     const int rc = launch_mfma_plan(J, s);

@@ -22,6 +22,15 @@ constexpr uint32_t kMcChanPerWave = 16 * kMcNB;
 // [r_begin, r_end, lo_0, hi_0, lo_1, hi_1, ...]; lo >= hi: no samples.
 constexpr uint32_t kPlanRings = 64;
 constexpr uint32_t kPlanStride = 2 + 2 * kPlanRings;
+// After the ring ranges, at kPlanHeadWords: the block's 64 cell windows
+// (Window, as block_cell_window gives them), then their unit vectors as
+// three arrays of 64 doubles (x, y, z): the gridding kernel reads them
+// instead of evaluating the cells' trigonometry and its dependent loads.
+// kPlanTileWords uint32 per tile in all.
+constexpr uint32_t kPlanHeadWords = (kPlanStride + 3u) & ~3u;
+constexpr uint32_t kPlanWinWords = kMcCols * (uint32_t)sizeof(Window) / 4u;
+constexpr uint32_t kPlanTileWords = kPlanHeadWords + kPlanWinWords + kMcCols * 3u * 2u;
+static_assert(sizeof(Window) % 8 == 0 && kPlanHeadWords % 2 == 0 && kPlanTileWords % 4 == 0, "plan alignment");
 
 // The weight form of a job (MfmaJob::kt): the kernel type, or
 // kKtGaussSmall (GAUSS1D whose radius rules out the exponent's asin branch
@@ -46,7 +55,7 @@ struct MfmaJob {
   float* wsum;        // [side * side] or NULL
   uint32_t kt, n_pad, nchan, dtype;  // kt: mfma_weight_form
   uint32_t tc_shift, tiles_x, ntiles, normalize;
-  uint32_t* plan;     // [ntiles][kPlanStride], or NULL (search in the gridding kernel)
+  uint32_t* plan;     // [ntiles][kPlanTileWords], or NULL (windows and searches in the gridding kernel)
   // 0: the MFMA contraction; else a one-channel job on the FP64 tile path
   // (one_channel_tiles) with tiles_x and ntiles counting kTileCols x
   // kTileRows tiles: this many tiles per work item, one per wave, when

@@ -182,7 +182,8 @@ class TorchBackend:
         sorted_values = r.nchan * n_pad * 2 + 256
         cube = r.nchan * m * 4 + m * 4
         tiles = -(-r.target.side // 8) * -(-r.target.side // 8)
-        plan = 4096 + tiles * 520 + 16 * tiles * max(1, -(-r.nchan // (64 * max(1, waves))))
+        # tile plans (3600 bytes per 64-cell tile) and work items (16 bytes per tile and channel group)
+        plan = 4096 + tiles * 3600 + 16 * tiles * max(1, -(-r.nchan // (64 * max(1, waves))))
         coords = 16 * r.n
         # host values land on the device in their own type before the conversion
         upload = 0 if r.on_device else r.values.size * r.values.itemsize

@@ -263,14 +263,15 @@ typedef struct hcg_mfma_shape {
 } hcg_mfma_shape;
 
 /* Workspace bytes of hcg_grid_mfma_ex for a side x side map and a launch
- * shape (NULL = defaults): the tile plan, 520 bytes per 64-cell tile. */
+ * shape (NULL = defaults): the tile plan, 3600 bytes per 64-cell tile. */
 size_t hcg_grid_mfma_workspace_size(uint32_t side, const hcg_mfma_shape *shape);
 
 /* hcg_grid_mfma with a launch shape (NULL = defaults) and a workspace for
  * the tile plan (NULL = none).  With a workspace, a first small kernel (one
- * wave per tile, many per CU) computes every tile's ring span and each
- * ring's sorted range, and the gridding kernel reads them instead of
- * searching the keys itself; the results are the same either way. */
+ * wave per tile, many per CU) computes every tile's ring span, each ring's
+ * sorted range and each cell's window and unit vector, and the gridding
+ * kernel reads them instead of evaluating them itself; the results are the
+ * same either way. */
 int hcg_grid_mfma_ex(const hcg_healpix *hp, const hcg_kernel *kern, uint32_t side, uint32_t n,
                      uint32_t nchan, int dtype, const hcg_sample *records, const uint32_t *keys_sorted,
                      const double *lon_sorted, const double *lat_sorted, const double *weights_sorted,
