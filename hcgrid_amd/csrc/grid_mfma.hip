@@ -222,6 +222,10 @@ struct MfmaLds {
   uint32_t dring[2][CHUNK / 32];       // per segment (at most CHUNK / 32 per chunk): its ring,
   uint16_t du0[2][CHUNK / 32], dnb[2][CHUNK / 32];  // first value block, value blocks
   uint32_t band[CHUNK / 32][kMcCols];  // each cell's band per segment: ja | jb << 16
+  // a planned tile's rings (the wave-parallel descriptor builder), slot l =
+  // ring r_begin + l, written and read by lane l of wave 0 only: first value
+  // block, the ring's offset in the tile's block sequence, its block count
+  uint32_t rl_b0[kPlanRings], rl_ex[kPlanRings], rl_nb[kPlanRings];
   Window win[kMcCols];                 // each cell's window
 };
 
@@ -426,6 +430,78 @@ __device__ __forceinline__ void mfma_block(const MfmaJob& J, uint32_t tile, uint
     }
   };
 
+  // The same packing built by the 64 lanes of wave 0 at once, for a tile
+  // whose rings are all in its plan: lane l holds ring r_begin + l (its
+  // block count nb, 0 when empty, and its offset ex in the tile's block
+  // sequence, an exclusive prefix sum), and the non-empty rings are ranked
+  // by popcount.  A chunk starting at block cpos ends at E = min(cpos + KV,
+  // the first k-block boundary at or after the end of the (KSEG - VPK +
+  // 1)-th non-empty ring from the one holding cpos) -- where the walk above
+  // stops -- and every ring overlapping [cpos, E) writes its own segment.
+  // Lanes exchange values by shuffles only: LDS slots are read back by the
+  // lane that wrote them.
+  const bool par_desc = J.plan != nullptr && r_begin <= r_end && r_end - r_begin < kPlanRings;  // uniform
+  uint32_t cpos = 0;
+  auto ring_list = [&]() {  // wave 0
+    const uint32_t r = r_begin + lane;
+    uint32_t nb = 0, b0 = 0;
+    if (r <= r_end) {
+      const uint32_t lo = plan_lo(r), hi = plan_hi(r);
+      if (lo < hi) {
+        b0 = lo / kVB;
+        nb = (hi + kVB - 1) / kVB - b0;
+      }
+    }
+    uint32_t incl = nb;  // inclusive prefix sum over the lanes
+#pragma unroll
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+      const uint32_t t = __shfl_up(incl, off, 64);
+      if (lane >= off) incl += t;
+    }
+    L.rl_b0[lane] = b0;
+    L.rl_ex[lane] = incl - nb;
+    L.rl_nb[lane] = nb;
+  };
+  auto build_desc_par = [&](uint32_t db) {  // wave 0, after ring_list
+    const uint32_t b0 = L.rl_b0[lane], ex = L.rl_ex[lane], nb = L.rl_nb[lane];  // this lane's own slots
+    const uint32_t in = ex + nb;
+    const uint32_t total = __shfl(in, 63, 64);
+    uint32_t nvb = 0, nseg = 0;
+    const uint64_t hm = __ballot(nb > 0 && ex <= cpos && cpos < in);  // the ring holding cpos
+    if (cpos < total && hm) {  // uniform (the rings' [ex, ex + nb) tile [0, total): one lane holds cpos)
+      const uint64_t ne = __ballot(nb > 0);
+      const uint32_t rank = (uint32_t)__popcll(ne & ((1ull << lane) - 1ull));
+      const uint32_t kl = (uint32_t)__ffsll((long long)hm) - 1u;
+      const uint32_t rk = __shfl(rank, kl, 64);  // the rank of the ring holding cpos
+      const uint64_t tm = __ballot(nb > 0 && rank == rk + (KSEG - (VPK - 1)) - 1u);
+      const uint32_t tl = tm ? (uint32_t)__ffsll((long long)tm) - 1u : 63u;
+      const uint32_t end_t = __shfl(in, tl, 64);
+      const uint32_t pos_s = tm ? end_t : total;
+      const uint32_t e = min(min(cpos + KV, total), (pos_s + VPK - 1) / VPK * VPK);
+      const uint32_t nkb = (e - cpos + VPK - 1) / VPK;
+      const bool seg = nb > 0 && lane >= kl && ex < e && rank - rk < KSEG;
+      if (seg) {
+        const uint32_t sg = rank - rk;
+        const uint32_t a = max(ex, cpos), b = min(in, e);
+        const uint32_t du = a - cpos, g0 = b0 + (a - ex);
+        L.dring[db][sg] = r_begin + lane;
+        L.du0[db][sg] = (uint16_t)du;
+        L.dnb[db][sg] = (uint16_t)(b - a);
+        for (uint32_t t = 0; t < b - a; ++t) L.dgb[db][du + t] = g0 + t;
+        if (b == e)  // the last segment repeats its last block up to the k-block boundary
+          for (uint32_t p = e - cpos; p < nkb * VPK; ++p) L.dgb[db][p] = g0 + (b - a - 1);
+      }
+      nseg = (uint32_t)__popcll(__ballot(seg));
+      nvb = e - cpos;
+      cpos = e;
+    }
+    if (lane == 0) {
+      L.dn[db][0] = (nvb + VPK - 1) / VPK;
+      L.dn[db][1] = nseg;
+      L.dn[db][2] = nvb;
+    }
+  };
+
   // ---- per-chunk steps -------------------------------------------------
   // stage chunk positions [from, 32 nkb) of descriptor db into LDS (keys past
   // the samples and past the chunk's nvb value blocks are the empty key,
@@ -589,7 +665,14 @@ __device__ __forceinline__ void mfma_block(const MfmaJob& J, uint32_t tile, uint
   // contraction reads only weight tiles and channel values), so a chunk of
   // at most nthreads samples needs no staging of its own.
   uint32_t db = 0, staged = 0;
-  if (wave == 0) build_desc(0);
+  if (wave == 0) {
+    if (par_desc) {
+      ring_list();
+      build_desc_par(0);
+    } else {
+      build_desc(0);
+    }
+  }
   reset_spans(0);
   __syncthreads();
   for (;;) {
@@ -606,7 +689,12 @@ __device__ __forceinline__ void mfma_block(const MfmaJob& J, uint32_t tile, uint
     // channel values of the chunk's first k-block, in flight while the weights are computed
     vec bn[kMcNB];
     if (ch_base < nchan) load_b_rows(bn, kblock(db, 0), lofs);
-    if (wave == 0) build_desc(db ^ 1u);
+    if (wave == 0) {
+      if (par_desc)
+        build_desc_par(db ^ 1u);
+      else
+        build_desc(db ^ 1u);
+    }
     weigh(db, nseg);
     __syncthreads();  // B3
     const uint32_t nn = L.dn[db ^ 1u][0] * 32u, nvb_next = L.dn[db ^ 1u][2];
