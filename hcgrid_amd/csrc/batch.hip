@@ -308,7 +308,7 @@ int plan_batch(uint32_t njobs, const hcg_batch_job* jobs, uint32_t waves, uint32
       groups[j] = 1;
       units[j] = ceil_div_u32(p.tiles[j], p.tpi[j]);
       cost[j] = item_cost(g, kTileCols, 1) * p.tpi[j] / 8.0;
-    } else if (few_channel_tiles(P, g->kern.kernel_type, false, b.nchan)) {
+    } else if (few_channel_tiles(P, g->kern.kernel_type, false, b.nchan, g->n)) {
       // few channels on a large map: FP64 tile path, one tile per wave
       p.tcols[j] = kTileCols;
       p.tiles[j] = tile_count_of(side);

@@ -1229,7 +1229,7 @@ extern "C" int hcg_grid_mfma_ex(const hcg_healpix* hp, const hcg_kernel* kern, u
                     (uint32_t)(normalize ? 1 : 0), 1u};
     return launch_tiles_one_channel(P, B, O, s);
   }
-  if (few_channel_tiles(P, kern->kernel_type, weights_sorted != nullptr, nchan)) {
+  if (few_channel_tiles(P, kern->kernel_type, weights_sorted != nullptr, nchan, n)) {
     const TileOut O{static_cast<const uint16_t*>(values_sorted), cube, wsum, dtype == HCG_DTYPE_BF16 ? 1u : 0u,
                     (uint32_t)(normalize ? 1 : 0), nchan};
     return launch_tiles_few_channels(P, B, O, s);

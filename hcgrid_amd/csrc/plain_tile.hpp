@@ -137,20 +137,25 @@ inline uint32_t one_channel_streams(uint32_t side) {
   return tile_count_of(side) < kOneChannelMinTiles ? kOneChannelStreams : 1u;
 }
 
-// Few-channel jobs (2..kFewChannels channels, otherwise as one_channel_tiles,
-// at dot_distance_ok radii (their tile loop tests candidates on p.q), on
-// maps of at least kOneChannelMinTiles tiles) also take the tile loop:
-// FP64 weights, FP32 accumulation per channel.  The MFMA kernel pays a
-// fixed chain of barriers and global round trips per 64-cell block, which
-// on sparse large maps outweighs the per-pair work (300k samples x 16
-// channels onto 2048^2: 5.4 ms there, one block per ~40 us per workgroup);
-// the tile loop runs one wave per tile at several waves per SIMD.
-constexpr int kFewChannels = 16;
+// Few-channel jobs (2..kFewChannels channels on sparse maps, at most one
+// sample per kFewCellsPerSample cells, otherwise as one_channel_tiles, at
+// dot_distance_ok radii (their tile loop tests candidates on p.q), on maps
+// of at least kOneChannelMinTiles tiles) also take the tile loop: FP64
+// weights, FP32 accumulation per channel, one wave per tile at several
+// waves per SIMD.  On sparse maps the MFMA kernel's fixed chain of barriers
+// and global round trips per 64-cell block outweighs the per-pair work:
+// 300k samples x 2..8 channels onto 2048^2 (0.07 samples per cell) took
+// 1.96-2.10 ms here against 2.20-2.23 ms there.  With 16 channels, or at
+// 0.6-13 samples per cell, the MFMA kernel is the faster one (3M x 16 onto
+// 2048^2: 6.98 against 8.35 ms; 3.4M x 2 onto 512^2: 3.31 against 4.23 ms).
+constexpr int kFewChannels = 8;
+constexpr uint32_t kFewCellsPerSample = 10;
 constexpr uint32_t kFewCap = 128;  // staged samples per round
 constexpr size_t kFewTileLdsBytes = (size_t)kFewCap * (3 * sizeof(double) + sizeof(uint32_t) + kFewChannels * 2);
 
-inline bool few_channel_tiles(const GridParams& P, uint32_t kernel_type, bool weighted, uint32_t nchan) {
-  return nchan >= 2 && nchan <= (uint32_t)kFewChannels && kernel_type == HCG_GAUSS1D && !weighted &&
+inline bool few_channel_tiles(const GridParams& P, uint32_t kernel_type, bool weighted, uint32_t nchan, uint32_t n) {
+  return nchan >= 2 && nchan <= (uint32_t)kFewChannels &&
+         (uint64_t)n * kFewCellsPerSample <= (uint64_t)P.side * P.side && kernel_type == HCG_GAUSS1D && !weighted &&
          gauss_tiny_radius(P) && dot_distance_ok(P) && tile_count_of(P.side) >= kOneChannelMinTiles;
 }
 

@@ -234,11 +234,11 @@ int hcg_gather_channels(uint32_t n, uint32_t nchan, const uint32_t *perm, const 
  *    the launch shape is ignored).  Maps of at least 4096 tiles of 16 x 4
  *    cells (side >= 512) give each tile one wave; smaller maps split each
  *    tile's rings over 4 partial streams, summed in stream order;
- *  - 2..16 channels with GAUSS1D, no weights_sorted, a radius between
+ *  - 2..8 channels with GAUSS1D, no weights_sorted, a radius between
  *    ~0.018 deg (where the candidate test on the unit vectors' dot product
- *    is exact enough) and ~0.26 deg, on maps of at least 4096 tiles, run
- *    that loop too, with one FP32 sum per channel (weights rounded once to
- *    FP32; wsum is the FP64 weight sum);
+ *    is exact enough) and ~0.26 deg, on sparse maps of at least 4096 tiles
+ *    (n <= side * side / 10), run that loop too, with one FP32 sum per
+ *    channel (weights rounded once to FP32; wsum is the FP64 weight sum);
  *  - everything else runs the MFMA contraction (w~ = weight rounded to
  *    dtype as hi + lo).
  * Batch jobs take the same routes, so a job's cube is the same alone or

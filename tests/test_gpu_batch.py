@@ -30,7 +30,8 @@ JOBS = [
     (2500, 20, 16, "f16", 0.02, 10.0, 64),
     (4000, 28, 48, "bf16", 0.03, 75.0, 0),
     (12000, 512, 1, "bf16", 0.02, 20.0, 0),  # one channel on a 512^2 map: the FP64 tile path
-    (12000, 512, 16, "f16", 0.02, 45.0, 0),  # 16 channels on a 512^2 map: few-channel tiles
+    (12000, 512, 8, "f16", 0.02, 45.0, 0),  # 8 channels on a sparse 512^2 map: few-channel tiles
+    (12000, 512, 16, "bf16", 0.02, 50.0, 0),  # 16 channels there: the contraction
     (20000, 24, 40, "bf16", 0.02, 10.0, 0),  # ~35 samples per cell: the dense jobs' own 8-wave launch
 ]
 
