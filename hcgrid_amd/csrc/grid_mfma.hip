@@ -240,7 +240,11 @@ struct MfmaLds {
 // blocks share more of their neighbourhood, so each channel value is
 // re-read by fewer workgroups: (TR + d)(TC + d) / 64 reads per value for a
 // support of d cells, 16.8 for 1 x 64 and 7.7 for 4 x 16 at d = 13.
-template <int DT, uint32_t CHUNK>
+// STAGED: the cube stores go through LDS (the one-job kernel, where they
+// took the 2M x 1024 cube from 7.29 to 7.08 ms; in the persistent kernel,
+// whose register budget is tighter, the 64-job batch measured 159.4 ms
+// with them against 158.9 without).
+template <int DT, uint32_t CHUNK, bool STAGED>
 __device__ __forceinline__ void mfma_block(const MfmaJob& J, uint32_t tile, uint32_t group, MfmaLds<CHUNK>& L,
                                            const double* s_exp) {
   typedef s16x8_t vec;
@@ -740,10 +744,49 @@ __device__ __forceinline__ void mfma_block(const MfmaJob& J, uint32_t tile, uint
   // lane l holds cells 16 mb + 4 (l / 16) + j, j = 0..3: four consecutive
   // cells of one map row (tile widths are multiples of 4), stored as one
   // 16-byte piece when the map side is a multiple of 4 (then aligned)
+  auto value = [&](int mb, int nb, int j, uint32_t cl0) {
+    return (float)((double)acc[mb][nb][j] * s_inv[cl0 + j]);
+  };
+  constexpr uint32_t kEpiFloats = kMcCols * 64;  // one wave's 64 cells x 64 channels
+  if (STAGED && (P.side & 3u) == 0 && nw * kEpiFloats * sizeof(float) <= sizeof(L.w)) {  // uniform
+    // Through LDS: each wave stages its cells x channels by channel (the
+    // 16-byte quads of a channel's row XOR-swizzled by the channel, so the
+    // 16 channels of a write, and the 16 quads of a read, fall in 16
+    // distinct bank groups), then stores four channels per instruction, a
+    // channel's 64 cells as 16 pieces of 4 consecutive cells of a map row.
+    // An instruction then spans 4 channel planes instead of 16 (each plane
+    // its own pages: 4 MB at 1024^2), so it needs fewer address
+    // translations.  The weight tiles are free: the last chunk's
+    // contraction and the weight-sum reduction are behind barriers.
+    float* T = reinterpret_cast<float*>(L.w) + (size_t)wave * kEpiFloats;
+    if (ch_base < nchan) {
+#pragma unroll
+      for (int mb = 0; mb < 4; ++mb) {
+        const uint32_t q = mb * 4 + (lane >> 4), cl0 = q * 4;
+#pragma unroll
+        for (int nb = 0; nb < kMcNB; ++nb) {
+          const uint32_t cr = nb * 16 + (lane & 15);
+          *reinterpret_cast<float4*>(&T[cr * 64 + ((q ^ (cr & 15u)) << 2)]) =
+              make_float4(value(mb, nb, 0, cl0), value(mb, nb, 1, cl0), value(mb, nb, 2, cl0), value(mb, nb, 3, cl0));
+        }
+      }
+    }
+    __syncthreads();
+    if (ch_base < nchan) {
+      const uint32_t q = lane & 15, cl0 = q * 4;
+      const uint32_t rowj = y0 + (cl0 >> tc_shift), col0 = x0 + (cl0 & tc_mask);
+      const bool in_map = col0 < P.side && rowj < P.side;
+#pragma unroll
+      for (uint32_t k = 0; k < 16; ++k) {
+        const uint32_t cr = k * 4 + (lane >> 4), ch = ch_base + cr;
+        const float4 v = *reinterpret_cast<const float4*>(&T[cr * 64 + ((q ^ (cr & 15u)) << 2)]);
+        if (in_map && ch < nchan)
+          *reinterpret_cast<float4*>(cube + (size_t)ch * M + (size_t)rowj * P.side + col0) = v;
+      }
+    }
+    return;
+  }
   if (ch_base < nchan) {
-    auto value = [&](int mb, int nb, int j, uint32_t cl0) {
-      return (float)((double)acc[mb][nb][j] * s_inv[cl0 + j]);
-    };
     if ((P.side & 3u) == 0) {  // uniform
 #pragma unroll
       for (int mb = 0; mb < 4; ++mb) {
@@ -790,7 +833,7 @@ __global__ void __launch_bounds__(64 * kMcMaxWaves) k_grid_mfma(MfmaJob J) {
   uint32_t tile = blockIdx.x;
   const uint32_t ntiles = J.ntiles;
   if ((ntiles & 7u) == 0) tile = (blockIdx.x & 7u) * (ntiles >> 3) + (blockIdx.x >> 3);
-  mfma_block<DT, CHUNK>(J, tile, blockIdx.y, L, s_exp);
+  mfma_block<DT, CHUNK, true>(J, tile, blockIdx.y, L, s_exp);
 }
 
 // Many jobs, one launch: a persistent grid pulls work items (job, tile,
@@ -904,9 +947,9 @@ __global__ void __launch_bounds__(64 * kMcMaxWaves)
     if (__builtin_amdgcn_readfirstlane(jobs[jb].tiles_per_item))
       tile_item(jobs[jb], tile, L, s_exp);
     else if (__builtin_amdgcn_readfirstlane(jobs[jb].dtype) == HCG_DTYPE_F16)
-      mfma_block<kDtF16, CHUNK>(jobs[jb], tile, group, L, s_exp);
+      mfma_block<kDtF16, CHUNK, false>(jobs[jb], tile, group, L, s_exp);
     else
-      mfma_block<kDtBf16, CHUNK>(jobs[jb], tile, group, L, s_exp);
+      mfma_block<kDtBf16, CHUNK, false>(jobs[jb], tile, group, L, s_exp);
     __syncthreads();  // LDS and s_item are free for the next item
   }
 }
