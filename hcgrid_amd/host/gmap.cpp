@@ -14,6 +14,7 @@
 #include "hcgrid_amd.h"
 #include "hcgrid_host.h"
 #include "host_internal.hpp"
+#include "wcs.hpp"
 
 // ---- module state (HCGrid/gmap.cpp:46-61, HCGrid/healpix.cu:39) ----
 double *h_lons = nullptr, *h_lats = nullptr, *h_data = nullptr, *h_weights = nullptr;
@@ -253,6 +254,10 @@ void read_output_map(const char* infile) {
 
 // ---- target coordinates (HCGrid/gmap.cpp:183-250) ----
 
+static_assert(hcgwcs::SIN == HCG_PROJ_SIN && hcgwcs::TAN == HCG_PROJ_TAN && hcgwcs::ARC == HCG_PROJ_ARC &&
+                  hcgwcs::ZEA == HCG_PROJ_ZEA && hcgwcs::STG == HCG_PROJ_STG,
+              "the zenithal codes are the C ABI's");
+
 int set_WCS() {
   clear_error();
   if (!h_header || !h_zyx) {
@@ -261,34 +266,10 @@ int set_WCS() {
   }
   std::vector<std::string> cards;
   for (size_t k = 0; k + 80 <= strlen(h_header); k += 80) cards.emplace_back(h_header + k, 80);
-  std::string ct1, ct2;
-  double crpix1, crpix2, cdelt1, cdelt2, crval1, crval2, lonpole = 180.0;
-  if (!hcgio::card_string(cards, "CTYPE1", ct1) || !hcgio::card_string(cards, "CTYPE2", ct2) ||
-      !hcgio::card_double(cards, "CRPIX1", crpix1) || !hcgio::card_double(cards, "CRPIX2", crpix2) ||
-      !hcgio::card_double(cards, "CDELT1", cdelt1) || !hcgio::card_double(cards, "CDELT2", cdelt2) ||
-      !hcgio::card_double(cards, "CRVAL1", crval1) || !hcgio::card_double(cards, "CRVAL2", crval2)) {
-    fail("set_WCS: target header lacks CTYPE/CRPIX/CDELT/CRVAL for axes 1 and 2");
-    return 0;
-  }
-  // zenithal projections, native pole at CRVAL: the default LONPOLE is 0
-  // when CRVAL2 reaches the pole and 180 otherwise (Calabretta & Greisen
-  // 2002, Paper II sec. 2.4, theta_0 = 90)
-  if (!hcgio::card_double(cards, "LONPOLE", lonpole)) lonpole = crval2 >= 90.0 ? 0.0 : 180.0;
-  static const char* const kProj[] = {"-SIN", "-TAN", "-ARC", "-ZEA", "-STG"};
-  uint32_t proj = 5;
-  for (uint32_t k = 0; k < 5; ++k)
-    if (ct1.size() >= 8 && ct2.size() >= 8 && ct1.substr(4) == kProj[k] && ct2.substr(4) == kProj[k]) proj = k;
-  const bool lon_first = proj < 5 &&
-                         (ct1.substr(0, 4) == "RA--" || ct1.substr(0, 4) == "GLON" || ct1.substr(0, 4) == "ELON") &&
-                         (ct2.substr(0, 4) == "DEC-" || ct2.substr(0, 4) == "GLAT" || ct2.substr(0, 4) == "ELAT");
-  double pc12 = 0, pc21 = 0, cd = 0, pv21 = 0, pv22 = 0;
-  const bool has_rot = hcgio::card_double(cards, "PC1_2", pc12) | hcgio::card_double(cards, "PC2_1", pc21) |
-                       hcgio::card_double(cards, "CD1_1", cd);
-  // SIN's slant parameters (Paper II eq. 38); the other four take none
-  const bool has_pv = hcgio::card_double(cards, "PV2_1", pv21) | hcgio::card_double(cards, "PV2_2", pv22);
-  if (!lon_first || (has_rot && (pc12 != 0 || pc21 != 0 || cd != 0)) || (has_pv && (pv21 != 0 || pv22 != 0))) {
-    fail("set_WCS: only zenithal targets (SIN, TAN, ARC, ZEA, STG) with longitude on axis 1, a diagonal CDELT "
-         "and no PV2 parameters are supported (got " + ct1 + ", " + ct2 + ")");
+  hcgwcs::Celestial w;
+  std::string err;
+  if (!hcgwcs::from_header(cards, w, err)) {
+    fail(err);
     return 0;
   }
   const uint32_t y = h_zyx[1], x = h_zyx[2];
@@ -296,9 +277,21 @@ int set_WCS() {
   free(h_ywcs);
   h_xwcs = (double*)malloc(sizeof(double) * (size_t)x * y);
   h_ywcs = (double*)malloc(sizeof(double) * (size_t)x * y);
-  if (!check_abi(hcg_wcs_p2s(proj, x, y, crpix1, crpix2, cdelt1, cdelt2, crval1, crval2, lonpole, h_xwcs, h_ywcs),
-                 "set_WCS"))
+  if (!h_xwcs || !h_ywcs) {
+    fail("set_WCS: malloc() failed");
     return 0;
+  }
+  // a zenithal target with a diagonal CDELT: the C ABI's form (the one the
+  // device entry point mirrors); anything else: the general host form
+  if (hcgwcs::plain_zenithal(w)) {
+    if (!check_abi(hcg_wcs_p2s((uint32_t)w.proj, x, y, w.crpix[0], w.crpix[1], w.m[0], w.m[3], w.crval[0],
+                               w.crval[1], w.lonpole, h_xwcs, h_ywcs),
+                   "set_WCS"))
+      return 0;
+  } else if (!hcgwcs::p2s(w, x, y, h_xwcs, h_ywcs, err)) {
+    fail(err);
+    return 0;
+  }
   return 1;
 }
 

@@ -18,7 +18,10 @@
  *    return early and hcg_host_error() reports the failure;
  *  - write_ordered_map writes the sorted coordinates as the 2 x N image the
  *    input uses (the reference calls fits_write_col on an image HDU);
- *  - only SIN (orthographic) celestial targets are supported by set_WCS.
+ *  - set_WCS maps the celestial targets its own formulas cover (wcs.cpp:
+ *    SIN with slant and NCP, TAN, ARC, ZEA, STG, CAR, MER, CEA, SFL, GLS,
+ *    AIT; CDELT, PC, CD or CROTA2; LONPOLE and LATPOLE) where the reference
+ *    takes whatever wcslib parses; other headers are an error.
  */
 #ifndef HCGRID_HOST_H
 #define HCGRID_HOST_H

@@ -114,6 +114,9 @@ def lib():
         L.orc_wcs_zenithal.restype = None
         L.orc_wcs_zenithal.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32] + [C.c_double] * 7 + [vp, vp]
         L.orc_wcs_sin.argtypes = [C.c_uint32, C.c_uint32] + [C.c_double] * 6 + [vp, vp]
+        L.orc_wcs_celestial.restype = C.c_int
+        L.orc_wcs_celestial.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_double, C.c_double, vp] + \
+            [C.c_double] * 6 + [vp, vp]
         _lib = L
     return _lib
 
@@ -345,6 +348,24 @@ def wcs_sin(naxis1, naxis2, crpix1, crpix2, cdelt1, cdelt2, crval1, crval2):
     yw = np.empty(naxis1 * naxis2)
     lib().orc_wcs_sin(naxis1, naxis2, crpix1, crpix2, cdelt1, cdelt2, crval1, crval2, _p(xw), _p(yw))
     return xw, yw
+
+
+WCS_CELESTIAL = dict(WCS_PROJ, CAR=5, MER=6, CEA=7, SFL=8, GLS=9, AIT=10)
+
+
+def wcs_celestial(proj, naxis1, naxis2, crpix1, crpix2, m, crval1, crval2, lonpole=None, latpole=90.0,
+                  pv=(0.0, 0.0)):
+    """General celestial WCS (orc_wcs_celestial): m is the 2x2 matrix of
+    degrees per pixel (CD, or CDELT_i PC_ij); lonpole None takes the
+    default; pv = (PV2_1, PV2_2).  Returns (lon, lat) or None when the
+    LONPOLE admits no native pole."""
+    xw = np.empty(naxis1 * naxis2)
+    yw = np.empty(naxis1 * naxis2)
+    mm = np.ascontiguousarray(np.asarray(m, dtype=np.float64).reshape(4))
+    rc = lib().orc_wcs_celestial(WCS_CELESTIAL[proj], naxis1, naxis2, crpix1, crpix2, _p(mm), crval1, crval2,
+                                 float("nan") if lonpole is None else lonpole, latpole, pv[0], pv[1],
+                                 _p(xw), _p(yw))
+    return None if rc else (xw, yw)
 
 
 def pipeline(lon, lat, data, xwcs, ywcs, side, sigma_deg, radius_deg, hpxres_deg, weights=None,

@@ -771,3 +771,163 @@ void orc_wcs_sin(uint32_t naxis1, uint32_t naxis2, double crpix1, double crpix2,
                  double *xwcs, double *ywcs) {
     orc_wcs_zenithal(0, naxis1, naxis2, crpix1, crpix2, cdelt1, cdelt2, crval1, crval2, 180.0, xwcs, ywcs);
 }
+
+/* The general celestial WCS of a 2-D target, what the reference's set_WCS
+ * gets from wcslib's wcsp2s (HCGrid/gmap.cpp:183-250), restated from
+ * Calabretta & Greisen 2002 (Paper II) in long double.
+ *   1. Intermediate world coordinates (degrees): (x, y) = m (p - crpix) with
+ *      p = (xx + 1, yy + 1) and m row-major (CD, or CDELT_i PC_ij).
+ *   2. Native (phi, theta) by the projection's inverse (X, Y = x, y in
+ *      radians): 0..4 the zenithal ones of orc_wcs_zenithal, SIN with the
+ *      slant (xi, eta) = (pv1, pv2): sin(theta) is the root nearest +1 of
+ *      a s^2 + 2 b s + c = 0 (a = xi^2 + eta^2 + 1, b = xi (X - xi) +
+ *      eta (Y - eta), c = (X - xi)^2 + (Y - eta)^2 - 1) and phi =
+ *      atan2(X - xi (1 - s), -(Y - eta (1 - s))) (sec. 5.1.5);
+ *      5 CAR phi = X, theta = Y; 6 MER theta = 2 atan(e^Y) - pi/2; 7 CEA
+ *      theta = asin(lambda Y), lambda = pv1; 8 SFL theta = Y, phi =
+ *      X / cos(theta); 10 AIT Z = sqrt(1 - (X/4)^2 - (Y/2)^2), phi =
+ *      2 atan2(X Z / 2, 2 Z^2 - 1), theta = asin(Y Z) (secs. 5.2.1-5.3.4).
+ *   3. Native -> celestial (eq. 2) about the celestial coordinates
+ *      (alpha_p, delta_p) of the native pole, phi_p = LONPOLE.  Zenithal:
+ *      (alpha_p, delta_p) = CRVAL.  theta_0 = 0 (5..8, 10): delta_p =
+ *      atan2(0, cos phi_p) +- acos(sin delta_0 / |cos phi_p|) (eq. 8),
+ *      the root in [-90, 90] nearest latpole, and alpha_p = alpha_0 -
+ *      atan2(sin phi_p, -sin delta_p cos phi_p) (eq. 2 at the reference
+ *      point).  lonpole NaN: 0 when delta_0 >= theta_0, else 180.
+ *   9 GLS (the AIPS convention, Greisen's AIPS memo 27): dec = crval2 + y,
+ *      ra = crval1 + x / cos(dec), no rotation.
+ * NaN outside the projection's domain.  Returns 0, or -1 when no delta_p
+ * satisfies eq. 8 (a LONPOLE the reference point cannot have). */
+int orc_wcs_celestial(uint32_t proj, uint32_t naxis1, uint32_t naxis2, double crpix1, double crpix2,
+                      const double *m, double crval1, double crval2, double lonpole, double latpole,
+                      double pv1, double pv2, double *xwcs, double *ywcs) {
+    const long double pi = 3.14159265358979323846264338327950288L;
+    const long double d2r = pi / 180.0L;
+    const int zenithal = proj <= 4, gls = proj == 9;
+    const long double a0 = crval1 * d2r, dl0 = crval2 * d2r;
+    long double php = isnan(lonpole) ? ((crval2 >= (zenithal ? 90.0 : 0.0)) ? 0.0L : pi) : lonpole * d2r;
+    long double ap = a0, dp = dl0;
+    if (!zenithal && !gls) {
+        const long double den = fabsl(cosl(php));
+        if (den < 1e-15L) {
+            if (fabsl(sinl(dl0)) > 1e-15L) return -1;
+            dp = latpole * d2r;
+        } else {
+            long double u = sinl(dl0) / den;
+            if (fabsl(u) > 1.0L + 1e-12L) return -1;
+            if (u > 1.0L) u = 1.0L;
+            if (u < -1.0L) u = -1.0L;
+            const long double base = atan2l(0.0L, cosl(php)), a = acosl(u);
+            long double best = NAN;
+            for (int k = 0; k < 2; ++k) {
+                long double c = k ? base - a : base + a;
+                if (c > pi) c -= 2 * pi;
+                if (c < -pi) c += 2 * pi;
+                if (fabsl(c) > pi / 2 + 1e-12L) continue;
+                if (c > pi / 2) c = pi / 2;
+                if (c < -pi / 2) c = -pi / 2;
+                if (isnan(best) || fabsl(c - latpole * d2r) < fabsl(best - latpole * d2r)) best = c;
+            }
+            if (isnan(best)) return -1;
+            dp = best;
+        }
+        ap = a0 - atan2l(sinl(php), -sinl(dp) * cosl(php));
+    }
+    for (uint32_t yy = 0; yy < naxis2; ++yy) {
+        for (uint32_t xx = 0; xx < naxis1; ++xx) {
+            const long double d1 = (long double)(xx + 1) - crpix1, d2 = (long double)(yy + 1) - crpix2;
+            const long double x = ((long double)m[0] * d1 + (long double)m[1] * d2) * d2r;
+            const long double y = ((long double)m[2] * d1 + (long double)m[3] * d2) * d2r;
+            const uint64_t c = (uint64_t)yy * naxis1 + xx;
+            long double phi = 0, theta = NAN;
+            if (gls) {
+                const long double dec = dl0 + y;
+                long double ra = NAN;
+                if (fabsl(dec) <= pi / 2) {
+                    const long double dl = cosl(dec) > 0 ? x / cosl(dec) : 0.0L;
+                    if (fabsl(dl) <= pi) ra = (a0 + dl) / d2r;
+                }
+                if (isnan(ra)) {
+                    xwcs[c] = NAN;
+                    ywcs[c] = NAN;
+                    continue;
+                }
+                while (ra < 0) ra += 360.0L;
+                while (ra >= 360.0L) ra -= 360.0L;
+                xwcs[c] = (double)ra;
+                ywcs[c] = (double)(dec / d2r);
+                continue;
+            }
+            switch (proj) {
+                case 0: {
+                    const long double xi = pv1, eta = pv2;
+                    const long double qa = xi * xi + eta * eta + 1.0L, qb = xi * (x - xi) + eta * (y - eta);
+                    const long double qc = (x - xi) * (x - xi) + (y - eta) * (y - eta) - 1.0L;
+                    const long double disc = qb * qb - qa * qc;
+                    if (disc < 0) break;
+                    long double s = (-qb + sqrtl(disc)) / qa;
+                    if (s > 1.0L + 1e-12L) s = (-qb - sqrtl(disc)) / qa;
+                    if (fabsl(s) > 1.0L + 1e-12L) break;
+                    if (s > 1.0L) s = 1.0L;
+                    theta = asinl(s);
+                    phi = atan2l(x - xi * (1.0L - s), -(y - eta * (1.0L - s)));
+                    break;
+                }
+                case 1: case 2: case 3: case 4: {
+                    const long double r = sqrtl(x * x + y * y);
+                    phi = atan2l(x, -y);
+                    if (proj == 1) theta = atan2l(1.0L, r);
+                    else if (proj == 2) theta = r <= pi ? pi / 2 - r : NAN;
+                    else if (proj == 3) theta = r <= 2.0L ? pi / 2 - 2 * asinl(r / 2) : NAN;
+                    else theta = pi / 2 - 2 * atanl(r / 2);
+                    break;
+                }
+                case 5:
+                    phi = x;
+                    theta = fabsl(y) <= pi / 2 ? y : NAN;
+                    break;
+                case 6:
+                    phi = x;
+                    theta = 2 * atanl(expl(y)) - pi / 2;
+                    break;
+                case 7: {
+                    const long double s = (long double)pv1 * y;
+                    phi = x;
+                    theta = fabsl(s) <= 1.0L ? asinl(s) : NAN;
+                    break;
+                }
+                case 8:
+                    if (fabsl(y) <= pi / 2) {
+                        phi = cosl(y) > 0 ? x / cosl(y) : 0.0L;
+                        theta = fabsl(phi) <= pi ? y : NAN;
+                    }
+                    break;
+                case 10: {
+                    const long double z2 = 1.0L - (x / 4) * (x / 4) - (y / 2) * (y / 2);
+                    if (z2 < 0.5L) break;
+                    const long double z = sqrtl(z2);
+                    phi = 2 * atan2l(x * z / 2, 2 * z2 - 1);
+                    theta = asinl(y * z);
+                    break;
+                }
+                default:
+                    break;
+            }
+            if (isnan(theta)) {
+                xwcs[c] = NAN;
+                ywcs[c] = NAN;
+                continue;
+            }
+            const long double dphi = phi - php;
+            const long double sd = sinl(theta) * sinl(dp) + cosl(theta) * cosl(dp) * cosl(dphi);
+            const long double dec = asinl(sd > 1 ? 1 : (sd < -1 ? -1 : sd));
+            long double ra = (ap + atan2l(-cosl(theta) * sinl(dphi),
+                                          sinl(theta) * cosl(dp) - cosl(theta) * sinl(dp) * cosl(dphi))) / d2r;
+            while (ra < 0) ra += 360.0L;
+            while (ra >= 360.0L) ra -= 360.0L;
+            xwcs[c] = (double)ra;
+            ywcs[c] = (double)(dec / d2r);
+        }
+    }
+    return 0;
+}

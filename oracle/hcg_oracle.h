@@ -164,6 +164,16 @@ void orc_wcs_sin(uint32_t naxis1, uint32_t naxis2, double crpix1, double crpix2,
                  double cdelt1, double cdelt2, double crval1, double crval2,
                  double *xwcs, double *ywcs);
 
+/* General celestial WCS (Calabretta & Greisen 2002): linear matrix m
+ * (row-major, degrees per pixel), projection proj (0 SIN with slant
+ * (pv1, pv2), 1 TAN, 2 ARC, 3 ZEA, 4 STG, 5 CAR, 6 MER, 7 CEA with
+ * lambda = pv1, 8 SFL, 9 GLS (AIPS), 10 AIT), native -> celestial rotation
+ * with phi_p = lonpole (NaN: the default) and delta_p chosen by latpole.
+ * Returns 0, or -1 when the LONPOLE admits no native pole. */
+int orc_wcs_celestial(uint32_t proj, uint32_t naxis1, uint32_t naxis2, double crpix1, double crpix2,
+                      const double *m, double crval1, double crval2, double lonpole, double latpole,
+                      double pv1, double pv2, double *xwcs, double *ywcs);
+
 #ifdef __cplusplus
 }
 #endif

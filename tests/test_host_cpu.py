@@ -172,19 +172,130 @@ def test_set_wcs_default_lonpole_at_the_pole(tmp_path):
     assert np.abs(xw - xo).max() < 1e-10 and np.abs(yw - yo).max() < 1e-10
 
 
-def test_set_wcs_rejects_other_projections(tmp_path):
-    lib, syms = _host()
-    p = _target_variant(tmp_path, "car.fits", (b"RA---SIN", b"RA---CAR"), (b"DEC--SIN", b"DEC--CAR"))
-    assert _set_wcs(lib, syms, p) == 0
-    assert b"only zenithal" in lib.hcg_host_error()
-    p = _target_variant(tmp_path, "mixed.fits", (b"DEC--SIN", b"DEC--TAN"))
-    assert _set_wcs(lib, syms, p) == 0
-    # SIN with a slant (PV2_1 != 0) is not the plain orthographic projection
+def _target_cards(tmp_path, name, set_cards=(), drop=()):
+    """target1.fits with cards replaced, added (set_cards: (key, value text))
+    or dropped; string values carry their quotes."""
     src = open(os.path.join(GOLDEN, "target1.fits"), "rb").read()
-    rd = src[src.index(b"RADESYS "):src.index(b"RADESYS ") + 80]
-    p = _target_variant(tmp_path, "slant.fits", (rd, _card("PV2_1", "0.1")))
-    assert _set_wcs(lib, syms, p) == 0
-    assert b"no PV2" in lib.hcg_host_error()
+    cards = [src[i:i + 80] for i in range(0, 2880, 80)]
+    end = next(k for k, c in enumerate(cards) if c.startswith(b"END"))
+    keys = {k for k, _ in set_cards} | set(drop) | {"CHECKSUM", "DATASUM"}
+    out = [c for c in cards[:end] if c[:8].decode().strip() not in keys]
+    out += [f"{k:<8}= {v:>20}".ljust(80).encode() for k, v in set_cards]
+    assert len(out) < 36
+    p = tmp_path / name
+    p.write_bytes(b"".join(out + [b"END".ljust(80)]).ljust(2880, b" ") + src[2880:])
+    return p
+
+
+def _wcs_arrays(lib, m=90 * 90):
+    xw = np.ctypeslib.as_array(C.cast(C.c_void_p.in_dll(lib, "h_xwcs").value, C.POINTER(C.c_double)), (m,))
+    yw = np.ctypeslib.as_array(C.cast(C.c_void_p.in_dll(lib, "h_ywcs").value, C.POINTER(C.c_double)), (m,))
+    return xw.copy(), yw.copy()
+
+
+def _close(got, ref, tol=1e-10):
+    fin = np.isfinite(ref[1])
+    assert np.array_equal(fin, np.isfinite(got[1])) and np.array_equal(fin, np.isfinite(got[0]))
+    dx = np.abs((got[0][fin] - ref[0][fin] + 180.0) % 360.0 - 180.0)
+    keep = np.abs(ref[1][fin]) < 89.99  # the longitude at a pole is arbitrary
+    assert dx[keep].max() < tol and np.abs(got[1][fin] - ref[1][fin]).max() < tol
+
+
+C1 = 0.055555555555556
+
+
+def _ctype(code):
+    return [("CTYPE1", f"'RA---{code}'"), ("CTYPE2", f"'DEC--{code}'")]
+
+
+# (projection, crval2, lonpole card or None, latpole, pv cards, oracle pv)
+GENERAL = [("CAR", 0.0, None, 90.0, [], (0.0, 0.0)), ("CAR", 30.0, None, 30.0, [], (0.0, 0.0)),
+           ("CAR", -35.0, None, -90.0, [], (0.0, 0.0)), ("MER", 20.0, None, 90.0, [], (0.0, 0.0)),
+           ("CEA", 10.0, None, 90.0, [("PV2_1", "0.6")], (0.6, 0.0)), ("CEA", -5.0, None, 90.0, [], (1.0, 0.0)),
+           ("SFL", 45.0, None, 90.0, [], (0.0, 0.0)), ("AIT", 60.0, None, 90.0, [], (0.0, 0.0)),
+           ("GLS", 30.0, None, 90.0, [], (0.0, 0.0)), ("AIT", 10.0, 0.0, 90.0, [], (0.0, 0.0)),
+           ("SIN", 30.0, 180.0, 30.0, [("PV2_1", "0.12"), ("PV2_2", "-0.2")], (0.12, -0.2)),
+           ("TAN", -60.0, 170.0, 30.0, [], (0.0, 0.0))]
+
+
+@pytest.mark.parametrize("proj,d0,lonpole,latpole,pvc,pv", GENERAL)
+def test_set_wcs_general_targets(tmp_path, proj, d0, lonpole, latpole, pvc, pv):
+    # each target against the oracle's general WCS at the same matrix, with
+    # a CDELT-only header and with a 20 deg PC rotation
+    lib, syms = _host()
+    lp = [("LONPOLE", repr(lonpole))] if lonpole is not None else []
+    base = _ctype(proj) + [("CRVAL2", repr(d0)), ("LATPOLE", repr(latpole))] + lp + pvc
+    drop = () if lonpole is not None else ("LONPOLE",)
+    p = _target_cards(tmp_path, "g.fits", base, drop)
+    assert _set_wcs(lib, syms, p) == 1, lib.hcg_host_error()
+    ref = O.wcs_celestial(proj, 90, 90, 45.0, 45.0, [-C1, 0, 0, C1], 60.0, d0, lonpole, latpole, pv)
+    _close(_wcs_arrays(lib), ref)
+    r = np.radians(20.0)
+    pc = [np.cos(r), -np.sin(r), np.sin(r), np.cos(r)]
+    p = _target_cards(tmp_path, "pc.fits", base + [(k, repr(float(v))) for k, v in zip(["PC1_1", "PC1_2", "PC2_1", "PC2_2"],
+                                                                                pc)], drop)
+    assert _set_wcs(lib, syms, p) == 1, lib.hcg_host_error()
+    ref = O.wcs_celestial(proj, 90, 90, 45.0, 45.0, [-C1 * pc[0], -C1 * pc[1], C1 * pc[2], C1 * pc[3]], 60.0, d0,
+                          lonpole, latpole, pv)
+    _close(_wcs_arrays(lib), ref)
+
+
+def test_set_wcs_linear_forms_agree(tmp_path):
+    # the same rotated grid written as PC + CDELT, as CD and as a legacy
+    # CROTA2 (Paper II eq. 188): one map
+    lib, syms = _host()
+    rho = 17.0
+    r = np.radians(rho)
+    c1, c2 = -C1, C1
+    pc = [np.cos(r), -np.sin(r) * c2 / c1, np.sin(r) * c1 / c2, np.cos(r)]
+    base = _ctype("CAR") + [("CRVAL2", "20.0"), ("LATPOLE", "90.0")]
+    forms = {"pc": [(k, repr(float(v))) for k, v in zip(["PC1_1", "PC1_2", "PC2_1", "PC2_2"], pc)],
+             "cd": [(k, repr(float(v))) for k, v in zip(["CD1_1", "CD1_2", "CD2_1", "CD2_2"],
+                                                 [c1 * pc[0], c1 * pc[1], c2 * pc[2], c2 * pc[3]])],
+             "crota": [("CROTA2", repr(rho))]}
+    maps = {}
+    for name, cards in forms.items():
+        drop = ("LONPOLE", "CDELT1", "CDELT2") if name == "cd" else ("LONPOLE",)
+        p = _target_cards(tmp_path, name + ".fits", base + cards, drop)
+        assert _set_wcs(lib, syms, p) == 1, (name, lib.hcg_host_error())
+        maps[name] = _wcs_arrays(lib)
+    ref = O.wcs_celestial("CAR", 90, 90, 45.0, 45.0, [c1 * pc[0], c1 * pc[1], c2 * pc[2], c2 * pc[3]], 60.0, 20.0)
+    for name in forms:
+        _close(maps[name], ref, 1e-11)
+
+
+@pytest.mark.parametrize("d0", [30.0, -65.0])
+def test_set_wcs_ncp_closed_form(tmp_path, d0):
+    # NCP: l = cos(dec) sin(ra - ra0), m = (cos(dec0) - cos(dec) cos(ra - ra0)) / sin(dec0)
+    lib, syms = _host()
+    p = _target_cards(tmp_path, "ncp.fits", _ctype("NCP") + [("CRVAL2", repr(d0))])
+    assert _set_wcs(lib, syms, p) == 1, lib.hcg_host_error()
+    xw, yw = _wcs_arrays(lib)
+    fin = np.isfinite(yw)
+    assert fin.sum() == fin.size
+    a, d, dd = np.radians(xw - 60.0), np.radians(yw), np.radians(d0)
+    lm_l = np.degrees(np.cos(d) * np.sin(a))
+    lm_m = np.degrees((np.cos(dd) - np.cos(d) * np.cos(a)) / np.sin(dd))
+    px = (np.arange(90 * 90) % 90) + 1 - 45.0
+    py = (np.arange(90 * 90) // 90) + 1 - 45.0
+    assert np.abs(lm_l - (-C1) * px).max() < 1e-10 and np.abs(lm_m - C1 * py).max() < 1e-10
+
+
+def test_set_wcs_rejects_what_it_cannot_map(tmp_path):
+    lib, syms = _host()
+    cases = [("mixed", [("CTYPE2", "'DEC--TAN'")], (), b"supported targets"),
+             ("latfirst", [("CTYPE1", "'DEC--SIN'"), ("CTYPE2", "'RA---SIN'")], (), b"supported targets"),
+             ("zpn", _ctype("ZPN"), (), b"supported targets"),
+             ("lonpole", _ctype("CAR") + [("LONPOLE", "180.0")], (), b"LONPOLE admits no native pole"),
+             ("cea", _ctype("CEA") + [("PV2_1", "1.5")], (), b"CEA needs"),
+             ("phi0", [("PV1_1", "10.0")], (), b"native reference point"),
+             ("singular", [("PC1_1", "1.0"), ("PC1_2", "1.0"), ("PC2_1", "1.0"), ("PC2_2", "1.0")], (), b"singular"),
+             ("nocdelt", [], ("CDELT1",), b"lacks CTYPE/CRPIX/CDELT/CRVAL"),
+             ("ncp0", _ctype("NCP") + [("CRVAL2", "0.0")], (), b"NCP needs")]
+    for name, cards, drop, msg in cases:
+        p = _target_cards(tmp_path, name + ".fits", cards, drop)
+        assert _set_wcs(lib, syms, p) == 0, name
+        assert msg in lib.hcg_host_error(), (name, lib.hcg_host_error())
 
 
 def test_write_output_map_roundtrip(tmp_path):
