@@ -63,6 +63,59 @@ def test_cli_on_reference_fixtures(gpu, tmp_path, order):
     assert np.array_equal(sc[n:], lat[perm].astype(np.float32).astype(np.float64))
 
 
+def _rewrite_target(src, dst, set_cards, drop=()):
+    """A copy of a one-block FITS target with header cards replaced or added."""
+    raw = open(src, "rb").read()
+    cards = [raw[i:i + 80] for i in range(0, 2880, 80)]
+    end = next(k for k, c in enumerate(cards) if c.startswith(b"END"))
+    keys = {k for k, _ in set_cards} | set(drop) | {"CHECKSUM", "DATASUM"}
+    out = [c for c in cards[:end] if c[:8].decode().strip() not in keys]
+    out += [f"{k:<8}= {v:>20}".ljust(80).encode() for k, v in set_cards]
+    with open(dst, "wb") as f:
+        f.write(b"".join(out + [b"END".ljust(80)]).ljust(2880, b" ") + raw[2880:])
+
+
+# (CTYPE code, extra cards, oracle matrix rotation in degrees, oracle pv)
+GENERAL_TARGETS = [("CAR", [("LATPOLE", "90.0")], 0.0, (0.0, 0.0)),
+                   ("AIT", [("LATPOLE", "90.0"), ("PC1_1", repr(math.cos(math.radians(30)))),
+                            ("PC1_2", repr(-math.sin(math.radians(30)))), ("PC2_1", repr(math.sin(math.radians(30)))),
+                            ("PC2_2", repr(math.cos(math.radians(30))))], 30.0, (0.0, 0.0)),
+                   ("SIN", [("PV2_1", "0.05"), ("PV2_2", "-0.1")], 0.0, (0.05, -0.1))]
+
+
+@pytest.mark.parametrize("code,cards,rot,pv", GENERAL_TARGETS)
+def test_cli_general_targets(gpu, tmp_path, code, cards, rot, pv):
+    # the reference's input on CAR, rotated AIT and slanted SIN targets: the
+    # drop-in's map against the oracle's pipeline on the oracle's WCS
+    shutil.copy(os.path.join(ROOT, "tests", "golden", "input1.fits"), tmp_path / "input1.fits")
+    lp = ("LONPOLE",) if code != "SIN" else ()
+    _rewrite_target(os.path.join(ROOT, "tests", "golden", "target1.fits"), str(tmp_path / "target1.fits"),
+                    [("CTYPE1", f"'RA---{code}'"), ("CTYPE2", f"'DEC--{code}'")] + cards, lp)
+    d = str(tmp_path) + "/"
+    print(f"case: HCGrid target {code}", flush=True)
+    p = subprocess.run([EXE, "--fits_path", d, "--input_file", "input", "--target_file", "target",
+                        "--output_file", "output", "--fits_id", "1", "--beam_size", "300", "--order_arg", "1"],
+                       capture_output=True, text=True, timeout=120)
+    print(p.stdout, p.stderr, flush=True)
+    assert p.returncode == 0
+    _, m = read_fits(str(tmp_path / "output1.fits"), 0)
+    kv, vals = read_fits(str(tmp_path / "input1.fits"), 0)
+    _, coords = read_fits(str(tmp_path / "input1.fits"), 1)
+    n = vals.size
+    lon, lat = coords[:n], coords[n:]
+    c1, c2 = -0.055555555555556, 0.055555555555556
+    r = math.radians(rot)
+    mat = [c1 * math.cos(r), -c1 * math.sin(r), c2 * math.sin(r), c2 * math.cos(r)]
+    xw, yw = O.wcs_celestial(code, 90, 90, 45.0, 45.0, mat, 60.0, 30.0, 180.0 if code == "SIN" else None, 90.0,
+                             pv)
+    sigma = (300.0 / 3600.0) / math.sqrt(8 * math.log(2))
+    ro = O.pipeline(lon, lat, vals, xw, yw, 90, sigma, 5 * sigma, sigma / 2)
+    e = map_errors(m, normalise(ro["datacube"], ro["weightscube"]))
+    print(e, flush=True)
+    assert e["nan_match"] and e["max_rel"] <= 1e-6 and e["cell_rel"] <= 1e-6
+    assert e["cells"] > 1000
+
+
 EXE_PLUS = os.path.join(ROOT, "hcgrid_amd", "HCGridPlus")
 
 
