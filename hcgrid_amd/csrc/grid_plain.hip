@@ -28,8 +28,9 @@
 //    <1e-17 relative for h < 1e-4 (radius < 1.1 deg), asin otherwise.
 //
 // Work decomposition: one 64-lane workgroup per tile of kTileCols x
-// (kLaneRows * kCellsPerLane) cells.  Lane l owns column l % kTileCols and
-// kCellsPerLane vertically adjacent cells.  The wave walks the union of its
+// (kLaneRows * kCellsPerLane) cells.  Lane l owns column l / kLaneRows and
+// kCellsPerLane vertically adjacent cells in it (column-major lanes,
+// plain_tile.hpp).  The wave walks the union of its
 // cells' ring ranges in ascending order; per non-empty ring it
 //   1. evaluates every cell's [lpix, rpix] window,
 //   2. finds the sorted-index range of the union window with a 64-way
