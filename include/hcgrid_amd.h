@@ -304,7 +304,10 @@ int hcg_eval_exp_neg(uint32_t n, const double *x, double *out, void *stream);
 /* Zenithal projections of the target map (Calabretta & Greisen 2002, Paper
  * II sec. 5.1).  The reference's set_WCS accepts whatever wcslib parses from
  * the target header (HCGrid/gmap.cpp:183-250); these cover the zenithal
- * family with the native pole at CRVAL. */
+ * family with a diagonal CDELT and the native pole at CRVAL.  The host
+ * layer's set_WCS (hcgrid_host.h) calls hcg_wcs_p2s for such targets and
+ * maps the other projections it supports (slanted SIN, NCP, cylindricals,
+ * SFL, GLS, AIT; PC, CD or CROTA2 matrices) on the host itself. */
 #define HCG_PROJ_SIN 0u  /* orthographic, PV2_1 = PV2_2 = 0 */
 #define HCG_PROJ_TAN 1u  /* gnomonic */
 #define HCG_PROJ_ARC 2u  /* zenithal equidistant */
